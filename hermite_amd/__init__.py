@@ -1,0 +1,204 @@
+"""MI355X-native Hermite force + predictor back end.
+
+The product is the C-ABI in ``include/hermite_hip.h`` implemented by
+``libhermite_hip.so`` (hand-written gfx950 kernels) and, above it, the C++
+drop-in ``Gravity`` (``include/hermite_hip_gravity.h``) and block-step driver.
+This module is a thin ctypes mirror of the same ``Gravity`` interface
+(``set_jp``, ``predict_all``, ``calc_force_on_first_nact``,
+``calc_potential``; reference ``SRC/hermite6-titan.h:4-133``) for tests and
+benchmarks.  It never falls back to a CPU path: without the built library or
+a gfx950 device every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .build import HIP_LIB, HOST_LIB, build  # noqa: F401
+
+PSTRIDE = {4: 24, 6: 28, 8: 36}   # reference Particle, doubles
+FSTRIDE = {4: 6, 6: 9, 8: 12}     # reference Force, doubles
+JPSTRIDE = {4: 14, 6: 20, 8: 26}  # device j-particle record, doubles
+FLOPS = {4: 60, 6: 97, 8: 144}    # conventional flops per interaction (SRC/hermite{4,6,8}.h)
+
+VARIANTS = {"fp64": 0, "split": 1, "exact": 2}
+
+_dp = C.POINTER(C.c_double)
+
+
+class HHError(RuntimeError):
+    pass
+
+
+_hip = None
+_host = None
+
+
+def hip_lib():
+    """Load libhermite_hip.so (raises if it has not been built)."""
+    global _hip
+    if _hip is None:
+        if not os.path.exists(HIP_LIB):
+            raise HHError(f"{HIP_LIB} is missing: run hermite_amd.build() (no CPU fallback exists)")
+        L = C.CDLL(HIP_LIB)
+        vp = C.c_void_p
+        L.hh_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]
+        L.hh_destroy.argtypes = [vp]
+        L.hh_set_jp.argtypes = [vp, C.c_int, _dp]
+        L.hh_jstore.argtypes = [vp]
+        L.hh_jstore.restype = _dp
+        L.hh_mark_dirty.argtypes = [vp, C.c_int, C.c_int]
+        L.hh_predict_all.argtypes = [vp, C.c_double]
+        L.hh_force_first_nact.argtypes = [vp, C.c_int, C.c_double, _dp, _dp]
+        L.hh_potential.argtypes = [vp, C.c_double, _dp]
+        L.hh_last_error.argtypes = [vp]
+        L.hh_last_error.restype = C.c_char_p
+        L.hh_set_tuning.argtypes = [vp, C.c_int, C.c_int]
+        L.hh_time_force.argtypes = [vp, C.c_int, C.c_double, C.c_int, _dp]
+        L.hh_time_predict.argtypes = [vp, C.c_double, C.c_int, _dp]
+        L.hh_force_async.argtypes = [vp, C.c_int, C.c_double, C.c_int]
+        L.hh_predict_async.argtypes = [vp, C.c_double]
+        L.hh_synchronize.argtypes = [vp]
+        L.hh_last_geometry.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        _hip = L
+    return _hip
+
+
+def host_lib():
+    global _host
+    if _host is None:
+        if not os.path.exists(HOST_LIB):
+            raise HHError(f"{HOST_LIB} is missing: run hermite_amd.build()")
+        L = C.CDLL(HOST_LIB)
+        L.hh_glibc_rand.argtypes = [C.c_uint, C.c_int, C.POINTER(C.c_int)]
+        L.hh_plummer.argtypes = [C.c_int, C.c_uint, C.c_double, _dp, _dp, _dp]
+        L.hh_salpeter.argtypes = [C.c_int, C.c_uint, C.c_double, C.c_double, C.c_double, _dp]
+        _host = L
+    return _host
+
+
+def _p(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+# ------------------------------------------------------------------ inputs
+def plummer(n: int, seed: int, rmax: float = 22.8):
+    """Plummer sphere (mass, pos, vel) in Henon units, RUN/gen-plum.c method."""
+    m = np.zeros(n)
+    x = np.zeros((n, 3))
+    v = np.zeros((n, 3))
+    host_lib().hh_plummer(n, seed, rmax, _p(m), _p(x), _p(v))
+    return m, x, v
+
+
+def salpeter(n: int, seed: int, alpha: float = 2.35, mlo: float = 0.1, mhi: float = 100.0):
+    m = np.zeros(n)
+    host_lib().hh_salpeter(n, seed, alpha, mlo, mhi, _p(m))
+    return m
+
+
+def make_particles(order: int, mass, pos, vel, ids=None, tlast: float = 0.0) -> np.ndarray:
+    """Reference Particle records (SRC/hermite6.h:37-47): derivatives zero."""
+    n = len(mass)
+    P = np.zeros((n, PSTRIDE[order]))
+    idv = np.arange(n, dtype=np.int64) if ids is None else np.asarray(ids, dtype=np.int64)
+    P[:, 0] = idv.view(np.float64)
+    P[:, 1] = mass
+    P[:, 2] = tlast
+    P[:, 4:7] = pos
+    P[:, 7:10] = vel
+    return P
+
+
+# ------------------------------------------------------------------ Gravity
+class Gravity:
+    """ctypes mirror of the reference's ``struct Gravity`` on the HIP back end."""
+
+    def __init__(self, order: int, nbody: int, variant: str = "fp64", device: int = 0):
+        L = hip_lib()
+        self.order, self.nbody, self.variant = order, nbody, variant
+        h = C.c_void_p()
+        rc = L.hh_create(order, nbody, VARIANTS[variant], device, C.byref(h))
+        if rc != 0:
+            raise HHError(f"hh_create({order}, {nbody}, {variant}) failed ({rc}): "
+                          f"{L.hh_last_error(None).decode()}")
+        self.h = h
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise HHError(f"{what} failed ({rc}): {hip_lib().hh_last_error(self.h).decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            hip_lib().hh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    # set_jp(addr, Particle) -------------------------------------------------
+    def set_jp(self, addr: int, rec: np.ndarray):
+        rec = np.ascontiguousarray(rec, dtype=np.float64)
+        self._check(hip_lib().hh_set_jp(self.h, addr, _p(rec)), "hh_set_jp")
+
+    def set_jp_all(self, P: np.ndarray):
+        """set_jp(i, P[i]) for every i, written straight into the pinned j-store."""
+        n = len(P)
+        js = np.ctypeslib.as_array(hip_lib().hh_jstore(self.h), shape=(self.nbody * JPSTRIDE[self.order],))
+        js = js.reshape(self.nbody, JPSTRIDE[self.order])
+        js[:n, 0:3] = P[:, 4:7]
+        js[:n, 3] = P[:, 1]
+        js[:n, 4:7] = P[:, 7:10]
+        js[:n, 7] = P[:, 2]
+        js[:n, 8:14] = P[:, 10:16]
+        if self.order >= 6:
+            js[:n, 14:20] = P[:, 16:22]
+        if self.order == 8:
+            js[:n, 20:26] = P[:, 22:28]
+        self._check(hip_lib().hh_mark_dirty(self.h, 0, n), "hh_mark_dirty")
+
+    def predict_all(self, tsys: float):
+        self._check(hip_lib().hh_predict_all(self.h, tsys), "hh_predict_all")
+
+    def calc_force_on_first_nact(self, nact: int, eps2: float, with_pot: bool = False):
+        F = np.zeros((nact, FSTRIDE[self.order]))
+        pot = np.zeros(nact) if with_pot else None
+        self._check(hip_lib().hh_force_first_nact(self.h, nact, eps2, _p(F), _p(pot) if with_pot else None),
+                    "hh_force_first_nact")
+        return (F, pot) if with_pot else F
+
+    def calc_potential(self, eps2: float) -> np.ndarray:
+        pot = np.zeros(self.nbody)
+        self._check(hip_lib().hh_potential(self.h, eps2, _p(pot)), "hh_potential")
+        return pot
+
+    # measurement -------------------------------------------------------------
+    def set_tuning(self, kernel: int = 0, jsplit: int = 0):
+        self._check(hip_lib().hh_set_tuning(self.h, kernel, jsplit), "hh_set_tuning")
+
+    def time_force(self, nact: int, eps2: float, reps: int = 10) -> float:
+        ms = C.c_double()
+        self._check(hip_lib().hh_time_force(self.h, nact, eps2, reps, C.byref(ms)), "hh_time_force")
+        return ms.value
+
+    def time_predict(self, tsys: float, reps: int = 10) -> float:
+        ms = C.c_double()
+        self._check(hip_lib().hh_time_predict(self.h, tsys, reps, C.byref(ms)), "hh_time_predict")
+        return ms.value
+
+    def predict_async(self, tsys: float):
+        self._check(hip_lib().hh_predict_async(self.h, tsys), "hh_predict_async")
+
+    def force_async(self, nact: int, eps2: float, with_pot: bool = False):
+        self._check(hip_lib().hh_force_async(self.h, nact, eps2, int(with_pot)), "hh_force_async")
+
+    def synchronize(self):
+        self._check(hip_lib().hh_synchronize(self.h), "hh_synchronize")
+
+    def last_geometry(self):
+        k, s, w = C.c_int(), C.c_int(), C.c_int()
+        hip_lib().hh_last_geometry(self.h, C.byref(k), C.byref(s), C.byref(w))
+        return {"kernel": k.value, "jsplit": s.value, "workgroups": w.value}

@@ -1,0 +1,512 @@
+// Device code of the Hermite force + predictor back end for gfx950 (CDNA4).
+//
+// Layout in HBM (one context = one Gravity instance, npad = nbody rounded up
+// to a multiple of 256, padded slots hold a massless particle far away):
+//   jst  : corrected j-particles, component-major (SoA): jst[w*npad + j] for
+//          the 14/20/26 words pos[3] mass vel[3] tlast acc[3] jrk[3]
+//          snp[3] crk[3] d4a[3] d5a[3] of the TITAN GParticle
+//          (SRC/hermite6-titan.h:20-29).
+//   jpr  : predicted j-particles, SoA: jpr[q*npad + j] for the 7/10/13 words
+//          pos[3] mass vel[3] acc[3] jrk[3] (SRC/hermite6-titan.h:31-36).
+//   part : per-j-slice partial forces part[(s*NC + c)*ipad + i].
+// This file is compiled with -ffp-contract=off: every fused multiply-add is
+// written explicitly, so the EXACT variant keeps the reference's rounding.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hh {
+
+enum Mode { FP64 = 0, SPLIT = 1, EXACT = 2 };
+
+constexpr int TJ = 256;     // j per LDS tile = threads per workgroup
+constexpr int NT = 256;     // threads per workgroup
+constexpr double FAR = 1.0e10;  // position of padding particles
+
+__host__ __device__ constexpr int jp_words(int o) { return o == 4 ? 14 : o == 6 ? 20 : 26; }
+__host__ __device__ constexpr int pr_words(int o) { return o == 4 ? 7 : o == 6 ? 10 : o == 8 ? 13 : 4; }
+__host__ __device__ constexpr int f_words(int o) { return o == 4 ? 6 : o == 6 ? 9 : o == 8 ? 12 : 0; }
+__host__ __device__ constexpr int p_words(int o) { return o == 4 ? 24 : o == 6 ? 28 : 36; }
+// words per j record in LDS, even so records can be read with 16-byte loads
+__host__ __device__ constexpr int lds_words(int o) { return o == 4 ? 8 : o == 6 ? 10 : o == 8 ? 14 : 4; }
+
+// ------------------------------------------------------------------ rsqrt
+// 1/sqrt(x) to FP64 accuracy: an FP32 v_rsq_f32 seed y0 refined with the
+// second-order truncation of (1-h)^(-1/2) = 1 + h/2 + 3h^2/8 + ..., where
+// h = 1 - x*y0^2 (|h| ~ 2^-23, so the dropped 5h^3/16 term is ~2^-70).
+// y0 has a 24-bit significand, so y0*y0 is exact in FP64 and h carries a
+// single rounding.  Valid for FLT_MIN < x < FLT_MAX.
+__device__ __forceinline__ double rsqrt64(double x)
+{
+    const float xf = (float)x;
+    const double y = (double)__builtin_amdgcn_rsqf(xf);
+    const double y2 = y * y;
+    const double h = __builtin_fma(-x, y2, 1.0);
+    const double p = __builtin_fma(h, 0.375, 0.5);
+    return __builtin_fma(y * h, p, y);
+}
+
+// --------------------------------------------------------------- predictor
+// Taylor prediction to tsys (Horner form of SRC/hermite6-titan.cu:153-227,
+// SRC/hermite8-titan.cu:20-134, generic SRC/hermite{4,6,8}.h GPredictor).
+// EXACT keeps the generic back end's unfused x + dt*(...) rounding.
+template <bool EX>
+__device__ __forceinline__ double madd(double a, double b, double c)
+{
+    if constexpr (EX) return c + a * b;
+    else return __builtin_fma(a, b, c);
+}
+
+// Source word w of slot i: the dirty staging record or the SoA store.
+template <int ORDER, bool EX>
+__device__ __forceinline__ void predict_words(const double (&r)[jp_words(ORDER)], double tsys, double (&q)[pr_words(ORDER)])
+{
+    const double dt = tsys - r[7];
+    const double dt2 = (1. / 2.) * dt;
+    const double dt3 = (1. / 3.) * dt;
+    const double dt4 = (1. / 4.) * dt;
+    const double dt5 = (1. / 5.) * dt;
+    const double dt6 = (1. / 6.) * dt;
+    const double dt7 = (1. / 7.) * dt;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const double x = r[0 + k], v = r[4 + k], a = r[8 + k], j = r[11 + k];
+        if constexpr (ORDER == 4) {
+            q[0 + k] = madd<EX>(dt, madd<EX>(dt2, madd<EX>(dt3, j, a), v), x);
+            q[4 + k] = madd<EX>(dt, madd<EX>(dt2, j, a), v);
+        } else if constexpr (ORDER == 6) {
+            const double s = r[14 + k], c = r[17 + k];
+            q[0 + k] = madd<EX>(dt, madd<EX>(dt2, madd<EX>(dt3, madd<EX>(dt4, madd<EX>(dt5, c, s), j), a), v), x);
+            q[4 + k] = madd<EX>(dt, madd<EX>(dt2, madd<EX>(dt3, madd<EX>(dt4, c, s), j), a), v);
+            q[7 + k] = madd<EX>(dt, madd<EX>(dt2, madd<EX>(dt3, c, s), j), a);
+        } else {
+            const double s = r[14 + k], c = r[17 + k], d4 = r[20 + k], d5 = r[23 + k];
+            q[0 + k] = madd<EX>(dt, madd<EX>(dt2, madd<EX>(dt3, madd<EX>(dt4, madd<EX>(dt5, madd<EX>(dt6, madd<EX>(dt7, d5, d4), c), s), j), a), v), x);
+            q[4 + k] = madd<EX>(dt, madd<EX>(dt2, madd<EX>(dt3, madd<EX>(dt4, madd<EX>(dt5, madd<EX>(dt6, d5, d4), c), s), j), a), v);
+            q[7 + k] = madd<EX>(dt, madd<EX>(dt2, madd<EX>(dt3, madd<EX>(dt4, madd<EX>(dt5, d5, d4), c), s), j), a);
+            q[10 + k] = madd<EX>(dt, madd<EX>(dt2, madd<EX>(dt3, madd<EX>(dt4, d5, d4), c), s), j);
+        }
+    }
+    q[3] = r[3];
+}
+
+// One thread per slot: slots below ndirty take their record from the
+// uploaded AoS staging buffer and write it into the SoA store (the upload of
+// SRC/hermite6-titan.cu:270 fused with the predict kernel :246-266); all
+// slots are then predicted to tsys.  predict == false only scatters.
+template <int ORDER, bool EX, bool PREDICT>
+__global__ __launch_bounds__(256) void predict_kernel(int npad, int ndirty, const double *__restrict__ stage,
+                                                      double *__restrict__ jst, double *__restrict__ jpr,
+                                                      double tsys)
+{
+    constexpr int W = jp_words(ORDER), Q = pr_words(ORDER);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npad) return;
+    double r[W];
+    if (i < ndirty) {
+#pragma unroll
+        for (int w = 0; w < W; w++) r[w] = stage[(size_t)i * W + w];
+#pragma unroll
+        for (int w = 0; w < W; w++) jst[(size_t)w * npad + i] = r[w];
+    } else {
+        if constexpr (!PREDICT) return;
+#pragma unroll
+        for (int w = 0; w < W; w++) r[w] = jst[(size_t)w * npad + i];
+    }
+    if constexpr (PREDICT) {
+        double q[Q];
+        predict_words<ORDER, EX>(r, tsys, q);
+#pragma unroll
+        for (int w = 0; w < Q; w++) jpr[(size_t)w * npad + i] = q[w];
+    }
+}
+
+// Fill the padding slots [nbody, npad) of the SoA store with massless
+// particles at (FAR, FAR, FAR) and everything else zero.
+template <int ORDER>
+__global__ void pad_kernel(int nbody, int npad, double *__restrict__ jst, double *__restrict__ jpr)
+{
+    constexpr int W = jp_words(ORDER), Q = pr_words(ORDER);
+    const int i = nbody + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npad) return;
+#pragma unroll
+    for (int w = 0; w < W; w++) jst[(size_t)w * npad + i] = (w < 3) ? FAR : 0.0;
+#pragma unroll
+    for (int w = 0; w < Q; w++) jpr[(size_t)w * npad + i] = (w < 3) ? FAR : 0.0;
+}
+
+// --------------------------------------------------------- force / potential
+struct TileArgs {
+    const double *__restrict__ src;  // SoA j source (jpr for forces, jst for the potential)
+    int npad;                        // component stride of src
+    int nbody;                       // real j count
+    int ntiles;                      // ceil(nbody / TJ)
+    int tiles_per_slice;
+    int nslice;
+    int nact;                        // i count (i-particles are src slots [0, nact))
+    int ipad;                        // i-blocks * i per block
+    double eps2;
+    int pot_mask_r2;                 // potential self-exclusion by r2 > eps2 (order 4)
+    double *__restrict__ part;
+    unsigned *__restrict__ cnt;
+    double *__restrict__ fout;       // [nact][F] (ORDER > 0) or [nact] (ORDER 0)
+    double *__restrict__ pout;       // fused potential [nact] or null
+};
+
+// Accumulator count: force words (+1 with the fused potential); ORDER 0 is
+// the stand-alone potential.
+__host__ __device__ constexpr int n_acc(int order, bool pot) { return order == 0 ? 1 : f_words(order) + (pot ? 1 : 0); }
+
+// ---- interaction bodies -------------------------------------------------
+// q: j record (pos[3] m vel[3] acc[3] jrk[3]); p: i record; A: accumulators
+// (acc[3] jrk[3] snp[3] crk[3] [pot]).  self: j is the i particle itself.
+
+// EXACT: operation order of the generic CPU back end, IEEE division and
+// square root (SRC/hermite4.h:347-357, SRC/hermite6.h:332-360,
+// SRC/hermite8.h:385-425; potential SRC/hermite4.h:373-384,
+// SRC/hermite6.h:376-386).
+template <int ORDER, bool POT>
+__device__ __forceinline__ void body_exact(const double *p, const double *q, double eps2, bool self, int mask_r2, double *A)
+{
+    if constexpr (ORDER == 0) {
+        const double dx = q[0] - p[0], dy = q[1] - p[1], dz = q[2] - p[2];
+        const double r2 = eps2 + (dx * dx + dy * dy + dz * dz);
+        const double mj = (mask_r2 ? (r2 > eps2) : !self) ? q[3] : 0.0;
+        A[0] -= mj / __builtin_sqrt(r2);
+        return;
+    } else {
+        double dr[3], dv[3], da[3], dj[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            dr[k] = q[0 + k] - p[0 + k];
+            dv[k] = q[4 + k] - p[4 + k];
+        }
+        const double r2 = eps2 + (dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
+        const double drdv = dr[0] * dv[0] + dr[1] * dv[1] + dr[2] * dv[2];
+        const double ri2 = 1.0 / r2;
+        const double sq = __builtin_sqrt(ri2);
+        const double mri3 = q[3] * ri2 * sq;
+        if constexpr (POT) {
+            const double mj = self ? 0.0 : q[3];
+            A[f_words(ORDER)] -= mj / __builtin_sqrt(r2);
+        }
+        if constexpr (ORDER == 4) {
+            const double alpha = -3.0 * drdv * ri2;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                A[k] = A[k] + mri3 * dr[k];
+                A[3 + k] = A[3 + k] + (mri3 * dv[k] + alpha * (mri3 * dr[k]));
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 3; k++) da[k] = q[7 + k] - p[7 + k];
+            const double dvdv = dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2];
+            const double drda = dr[0] * da[0] + dr[1] * da[1] + dr[2] * da[2];
+            const double alpha = drdv * ri2;
+            const double beta = (dvdv + drda) * ri2 + alpha * alpha;
+            if constexpr (ORDER == 6) {
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    A[k] = A[k] + mri3 * dr[k];
+                    const double tmp = dv[k] + (-3.0 * alpha) * dr[k];
+                    A[3 + k] = A[3 + k] + mri3 * tmp;
+                    A[6 + k] = A[6 + k] + mri3 * (da[k] + (-6.0 * alpha) * tmp + (-3.0 * beta) * dr[k]);
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < 3; k++) dj[k] = q[10 + k] - p[10 + k];
+                const double dvda = dv[0] * da[0] + dv[1] * da[1] + dv[2] * da[2];
+                const double drdj = dr[0] * dj[0] + dr[1] * dj[1] + dr[2] * dj[2];
+                const double gamma = (3.0 * dvda + drdj) * ri2 + alpha * (3.0 * beta - 4.0 * alpha * alpha);
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    A[k] = A[k] + mri3 * dr[k];
+                    const double t1 = dv[k] + (-3.0 * alpha) * dr[k];
+                    A[3 + k] = A[3 + k] + mri3 * t1;
+                    const double t2 = da[k] + (-6.0 * alpha) * t1 + (-3.0 * beta) * dr[k];
+                    A[6 + k] = A[6 + k] + mri3 * t2;
+                    const double t3 = dj[k] + (-9.0 * alpha) * t2 + (-9.0 * beta) * t1 + (-3.0 * gamma) * dr[k];
+                    A[9 + k] = A[9 + k] + mri3 * t3;
+                }
+            }
+        }
+    }
+}
+
+// FP64: same chain rule with explicit FMAs and the refined rsqrt
+// (the TITAN pp_interact of SRC/hermite{4,6,8}-titan.cu reorganised for
+// fused multiply-adds).
+template <int ORDER, bool POT>
+__device__ __forceinline__ void body_fp64(const double *p, const double *q, double eps2, bool self, int mask_r2, double *A)
+{
+    const double dx = q[0] - p[0], dy = q[1] - p[1], dz = q[2] - p[2];
+    const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, __builtin_fma(dx, dx, eps2)));
+    const double rinv = rsqrt64(r2);
+    if constexpr (ORDER == 0) {
+        const bool keep = mask_r2 ? (r2 > eps2) : !self;
+        A[0] = __builtin_fma(keep ? -q[3] : 0.0, rinv, A[0]);
+        return;
+    } else {
+        const double dvx = q[4] - p[4], dvy = q[5] - p[5], dvz = q[6] - p[6];
+        const double rinv2 = rinv * rinv;
+        const double mr = q[3] * rinv;
+        const double mr3 = mr * rinv2;
+        if constexpr (POT) A[f_words(ORDER)] = __builtin_fma(self ? 0.0 : -1.0, mr, A[f_words(ORDER)]);
+        const double drdv = __builtin_fma(dz, dvz, __builtin_fma(dy, dvy, dx * dvx));
+        A[0] = __builtin_fma(mr3, dx, A[0]);
+        A[1] = __builtin_fma(mr3, dy, A[1]);
+        A[2] = __builtin_fma(mr3, dz, A[2]);
+        if constexpr (ORDER == 4) {
+            const double a3 = -3.0 * (drdv * rinv2);
+            A[3] = __builtin_fma(mr3, __builtin_fma(a3, dx, dvx), A[3]);
+            A[4] = __builtin_fma(mr3, __builtin_fma(a3, dy, dvy), A[4]);
+            A[5] = __builtin_fma(mr3, __builtin_fma(a3, dz, dvz), A[5]);
+        } else {
+            const double dax = q[7] - p[7], day = q[8] - p[8], daz = q[9] - p[9];
+            const double alpha = drdv * rinv2;
+            const double dvdv_drda = __builtin_fma(dz, daz, __builtin_fma(dy, day, __builtin_fma(dx, dax,
+                                     __builtin_fma(dvz, dvz, __builtin_fma(dvy, dvy, dvx * dvx)))));
+            const double beta = __builtin_fma(dvdv_drda, rinv2, alpha * alpha);
+            const double a3 = -3.0 * alpha;
+            const double tx = __builtin_fma(a3, dx, dvx), ty = __builtin_fma(a3, dy, dvy), tz = __builtin_fma(a3, dz, dvz);
+            A[3] = __builtin_fma(mr3, tx, A[3]);
+            A[4] = __builtin_fma(mr3, ty, A[4]);
+            A[5] = __builtin_fma(mr3, tz, A[5]);
+            const double a6 = 2.0 * a3, b3 = -3.0 * beta;
+            const double ux = __builtin_fma(b3, dx, __builtin_fma(a6, tx, dax));
+            const double uy = __builtin_fma(b3, dy, __builtin_fma(a6, ty, day));
+            const double uz = __builtin_fma(b3, dz, __builtin_fma(a6, tz, daz));
+            A[6] = __builtin_fma(mr3, ux, A[6]);
+            A[7] = __builtin_fma(mr3, uy, A[7]);
+            A[8] = __builtin_fma(mr3, uz, A[8]);
+            if constexpr (ORDER == 8) {
+                const double djx = q[10] - p[10], djy = q[11] - p[11], djz = q[12] - p[12];
+                const double dvda = __builtin_fma(dvz, daz, __builtin_fma(dvy, day, dvx * dax));
+                const double drdj = __builtin_fma(dz, djz, __builtin_fma(dy, djy, dx * djx));
+                // gamma = (3 dv.da + dr.dj)/r^2 + alpha (3 beta - 4 alpha^2)
+                const double g0 = __builtin_fma(3.0, dvda, drdj);
+                const double g1 = __builtin_fma(-4.0 * alpha, alpha, 3.0 * beta);
+                const double gamma = __builtin_fma(g0, rinv2, alpha * g1);
+                const double a9 = 3.0 * a3, b9 = 3.0 * b3, g3 = -3.0 * gamma;
+                const double cx = __builtin_fma(g3, dx, __builtin_fma(b9, tx, __builtin_fma(a9, ux, djx)));
+                const double cy = __builtin_fma(g3, dy, __builtin_fma(b9, ty, __builtin_fma(a9, uy, djy)));
+                const double cz = __builtin_fma(g3, dz, __builtin_fma(b9, tz, __builtin_fma(a9, uz, djz)));
+                A[9] = __builtin_fma(mr3, cx, A[9]);
+                A[10] = __builtin_fma(mr3, cy, A[10]);
+                A[11] = __builtin_fma(mr3, cz, A[11]);
+            }
+        }
+    }
+}
+
+// SPLIT: relative position, velocity, acceleration and jerk formed in FP64
+// and rounded to FP32; r^-1 from v_rsq_f32; the rest of the chain rule in
+// FP32.  A holds FP32 partial sums, folded into FP64 by the caller.
+template <int ORDER, bool POT>
+__device__ __forceinline__ void body_split(const double *p, const double *q, float eps2, bool self, int mask_r2, float *A)
+{
+    const float dx = (float)(q[0] - p[0]), dy = (float)(q[1] - p[1]), dz = (float)(q[2] - p[2]);
+    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, eps2)));
+    const float rinv = __builtin_amdgcn_rsqf(r2);
+    const float mj = (float)q[3];
+    if constexpr (ORDER == 0) {
+        const bool keep = mask_r2 ? (r2 > eps2) : !self;
+        A[0] = __builtin_fmaf(keep ? -mj : 0.0f, rinv, A[0]);
+        return;
+    } else {
+        const float dvx = (float)(q[4] - p[4]), dvy = (float)(q[5] - p[5]), dvz = (float)(q[6] - p[6]);
+        const float rinv2 = rinv * rinv;
+        const float mr = mj * rinv;
+        const float mr3 = mr * rinv2;
+        if constexpr (POT) A[f_words(ORDER)] = __builtin_fmaf(self ? 0.0f : -1.0f, mr, A[f_words(ORDER)]);
+        const float drdv = __builtin_fmaf(dz, dvz, __builtin_fmaf(dy, dvy, dx * dvx));
+        A[0] = __builtin_fmaf(mr3, dx, A[0]);
+        A[1] = __builtin_fmaf(mr3, dy, A[1]);
+        A[2] = __builtin_fmaf(mr3, dz, A[2]);
+        if constexpr (ORDER == 4) {
+            const float a3 = -3.0f * (drdv * rinv2);
+            A[3] = __builtin_fmaf(mr3, __builtin_fmaf(a3, dx, dvx), A[3]);
+            A[4] = __builtin_fmaf(mr3, __builtin_fmaf(a3, dy, dvy), A[4]);
+            A[5] = __builtin_fmaf(mr3, __builtin_fmaf(a3, dz, dvz), A[5]);
+        } else {
+            const float dax = (float)(q[7] - p[7]), day = (float)(q[8] - p[8]), daz = (float)(q[9] - p[9]);
+            const float alpha = drdv * rinv2;
+            const float s = __builtin_fmaf(dz, daz, __builtin_fmaf(dy, day, __builtin_fmaf(dx, dax,
+                            __builtin_fmaf(dvz, dvz, __builtin_fmaf(dvy, dvy, dvx * dvx)))));
+            const float beta = __builtin_fmaf(s, rinv2, alpha * alpha);
+            const float a3 = -3.0f * alpha;
+            const float tx = __builtin_fmaf(a3, dx, dvx), ty = __builtin_fmaf(a3, dy, dvy), tz = __builtin_fmaf(a3, dz, dvz);
+            A[3] = __builtin_fmaf(mr3, tx, A[3]);
+            A[4] = __builtin_fmaf(mr3, ty, A[4]);
+            A[5] = __builtin_fmaf(mr3, tz, A[5]);
+            const float a6 = 2.0f * a3, b3 = -3.0f * beta;
+            const float ux = __builtin_fmaf(b3, dx, __builtin_fmaf(a6, tx, dax));
+            const float uy = __builtin_fmaf(b3, dy, __builtin_fmaf(a6, ty, day));
+            const float uz = __builtin_fmaf(b3, dz, __builtin_fmaf(a6, tz, daz));
+            A[6] = __builtin_fmaf(mr3, ux, A[6]);
+            A[7] = __builtin_fmaf(mr3, uy, A[7]);
+            A[8] = __builtin_fmaf(mr3, uz, A[8]);
+            if constexpr (ORDER == 8) {
+                const float djx = (float)(q[10] - p[10]), djy = (float)(q[11] - p[11]), djz = (float)(q[12] - p[12]);
+                const float dvda = __builtin_fmaf(dvz, daz, __builtin_fmaf(dvy, day, dvx * dax));
+                const float drdj = __builtin_fmaf(dz, djz, __builtin_fmaf(dy, djy, dx * djx));
+                const float g0 = __builtin_fmaf(3.0f, dvda, drdj);
+                const float g1 = __builtin_fmaf(-4.0f * alpha, alpha, 3.0f * beta);
+                const float gamma = __builtin_fmaf(g0, rinv2, alpha * g1);
+                const float a9 = 3.0f * a3, b9 = 3.0f * b3, g3 = -3.0f * gamma;
+                const float cx = __builtin_fmaf(g3, dx, __builtin_fmaf(b9, tx, __builtin_fmaf(a9, ux, djx)));
+                const float cy = __builtin_fmaf(g3, dy, __builtin_fmaf(b9, ty, __builtin_fmaf(a9, uy, djy)));
+                const float cz = __builtin_fmaf(g3, dz, __builtin_fmaf(b9, tz, __builtin_fmaf(a9, uz, djz)));
+                A[9] = __builtin_fmaf(mr3, cx, A[9]);
+                A[10] = __builtin_fmaf(mr3, cy, A[10]);
+                A[11] = __builtin_fmaf(mr3, cz, A[11]);
+            }
+        }
+    }
+}
+
+// j per FP32 partial sum in the SPLIT body before it is folded into FP64.
+constexpr int SPLIT_FOLD = 32;
+
+// ---- i-parallel tile kernel ----------------------------------------------
+// Workgroup (iblock, slice): NT threads each own NI i-particles
+// (i = iblock*NT*NI + k*NT + tid); the slice's j-tiles are staged through LDS
+// as j-major records (all lanes then read the same address: LDS broadcast),
+// the next tile's global loads in flight while the current tile is used.
+// With nslice > 1 every workgroup writes its partial sums; the last one to
+// finish an i-block (device-scope counter) adds the slices in slice order,
+// so the result does not depend on scheduling.  ORDER 0 is the potential.
+template <int ORDER, int MODE, int NI, bool POT>
+__global__ __launch_bounds__(NT) void tile_kernel(TileArgs a)
+{
+    constexpr int Q = pr_words(ORDER);
+    constexpr int L = lds_words(ORDER);
+    constexpr int NC = n_acc(ORDER, POT);
+    // i words actually used: pos (+ vel, acc, jrk); mass not needed
+    constexpr int QI = ORDER == 0 ? 3 : Q;
+    __shared__ double sj[TJ * L];
+    __shared__ unsigned s_last;
+
+    const int tid = threadIdx.x;
+    const int iblock = blockIdx.x, slice = blockIdx.y;
+    const int npad = a.npad;
+
+    double pi[NI][QI];
+    int ii[NI];
+#pragma unroll
+    for (int k = 0; k < NI; k++) {
+        ii[k] = iblock * NT * NI + k * NT + tid;
+        const int idx = ii[k] < a.nact ? ii[k] : a.nact - 1;
+#pragma unroll
+        for (int w = 0; w < QI; w++) pi[k][w] = a.src[(size_t)w * npad + idx];
+    }
+    double A[NI][NC];
+#pragma unroll
+    for (int k = 0; k < NI; k++)
+#pragma unroll
+        for (int c = 0; c < NC; c++) A[k][c] = 0.0;
+
+    const int t0 = slice * a.tiles_per_slice;
+    const int t1 = min(t0 + a.tiles_per_slice, a.ntiles);
+
+    double jr[Q];
+    if (t0 < t1) {
+#pragma unroll
+        for (int w = 0; w < Q; w++) jr[w] = a.src[(size_t)w * npad + t0 * TJ + tid];
+    }
+    const float eps2f = (float)a.eps2;
+
+    for (int t = t0; t < t1; t++) {
+        __syncthreads();
+#pragma unroll
+        for (int w = 0; w < Q; w++) sj[tid * L + w] = jr[w];
+        __syncthreads();
+        if (t + 1 < t1) {
+#pragma unroll
+            for (int w = 0; w < Q; w++) jr[w] = a.src[(size_t)w * npad + (t + 1) * TJ + tid];
+        }
+        const int jbase = t * TJ;
+        if constexpr (MODE == EXACT) {
+            const int jn = min(TJ, a.nbody - jbase);
+            for (int jj = 0; jj < jn; jj++) {
+                double q[L];
+#pragma unroll
+                for (int w = 0; w < L; w++) q[w] = sj[jj * L + w];
+#pragma unroll
+                for (int k = 0; k < NI; k++)
+                    body_exact<ORDER, POT>(pi[k], q, a.eps2, jbase + jj == ii[k], a.pot_mask_r2, A[k]);
+            }
+        } else if constexpr (MODE == FP64) {
+#pragma unroll 2
+            for (int jj = 0; jj < TJ; jj++) {
+                double q[L];
+#pragma unroll
+                for (int w = 0; w < L; w++) q[w] = sj[jj * L + w];
+#pragma unroll
+                for (int k = 0; k < NI; k++)
+                    body_fp64<ORDER, POT>(pi[k], q, a.eps2, jbase + jj == ii[k], a.pot_mask_r2, A[k]);
+            }
+        } else {
+            for (int jb = 0; jb < TJ; jb += SPLIT_FOLD) {
+                float Af[NI][NC];
+#pragma unroll
+                for (int k = 0; k < NI; k++)
+#pragma unroll
+                    for (int c = 0; c < NC; c++) Af[k][c] = 0.0f;
+#pragma unroll 2
+                for (int jj = jb; jj < jb + SPLIT_FOLD; jj++) {
+                    double q[L];
+#pragma unroll
+                    for (int w = 0; w < L; w++) q[w] = sj[jj * L + w];
+#pragma unroll
+                    for (int k = 0; k < NI; k++)
+                        body_split<ORDER, POT>(pi[k], q, eps2f, jbase + jj == ii[k], a.pot_mask_r2, Af[k]);
+                }
+#pragma unroll
+                for (int k = 0; k < NI; k++)
+#pragma unroll
+                    for (int c = 0; c < NC; c++) A[k][c] += (double)Af[k][c];
+            }
+        }
+    }
+
+    constexpr int F = ORDER == 0 ? 1 : f_words(ORDER);
+    if (a.nslice == 1) {
+#pragma unroll
+        for (int k = 0; k < NI; k++) {
+            if (ii[k] < a.nact) {
+#pragma unroll
+                for (int c = 0; c < F; c++) a.fout[(size_t)ii[k] * F + c] = A[k][c];
+                if constexpr (POT) a.pout[ii[k]] = A[k][F];
+            }
+        }
+        return;
+    }
+    const size_t ipad = (size_t)a.ipad;
+#pragma unroll
+    for (int k = 0; k < NI; k++)
+#pragma unroll
+        for (int c = 0; c < NC; c++) a.part[((size_t)slice * NC + c) * ipad + ii[k]] = A[k][c];
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) s_last = (atomicAdd(&a.cnt[iblock], 1u) == (unsigned)(a.nslice - 1));
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+#pragma unroll
+    for (int k = 0; k < NI; k++) {
+        if (ii[k] >= a.nact) continue;
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            double s = 0.0;
+            for (int sl = 0; sl < a.nslice; sl++)
+                s += __hip_atomic_load(&a.part[((size_t)sl * NC + c) * ipad + ii[k]], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            if (c < F) a.fout[(size_t)ii[k] * F + c] = s;
+            else a.pout[ii[k]] = s;
+        }
+    }
+    if (tid == 0) a.cnt[iblock] = 0u;
+}
+
+}  // namespace hh

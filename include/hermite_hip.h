@@ -1,0 +1,140 @@
+/*
+ * hermite_hip.h - C-ABI of the MI355X (gfx950) Hermite force + predictor
+ * back end.
+ *
+ * The entry points replace the methods of the reference's compile-time force
+ * back end `struct Gravity` (one per Hermite order), which the block-step
+ * driver calls through exactly five methods (SRC/nbodysystem.h:308,326,569,
+ * 575-576,583,588,635,703,707,731,755):
+ *
+ *   Gravity(int nbody) / ~Gravity()      -> hh_create / hh_destroy
+ *   set_jp(int addr, const Particle &p)  -> hh_set_jp
+ *   predict_all(double tsys)             -> hh_predict_all
+ *   calc_force_on_first_nact(nact, eps2, Force force[])
+ *                                        -> hh_force_first_nact
+ *   calc_potential(eps2, double potbuf[])-> hh_potential
+ *
+ * Reference interfaces replaced (TITAN back end): SRC/hermite4-titan.h:4-114,
+ * SRC/hermite6-titan.h:4-133, SRC/hermite8-titan.h:4-148 and the host code of
+ * SRC/hermite{4,6,8}-titan.cu and SRC/pot-titan.hu:131-171.
+ *
+ * Records crossing the boundary use the reference's own host layouts:
+ *   Particle : SRC/hermite4.h:16-31 (24 doubles), SRC/hermite6.h:19-35 (28),
+ *              SRC/hermite8.h:26-45 (36); word 0 is the int64 id.
+ *   Force    : acc, jrk[, snp[, crk]] = 6 / 9 / 12 doubles
+ *              (SRC/hermite4.h:4-14, SRC/hermite6.h:4-17, SRC/hermite8.h:8-24).
+ *
+ * Every function returns HH_OK (0) or a negative HH_E* code; the message of
+ * the last failure on a context is available from hh_last_error().  No
+ * function aborts the process (the reference's checkCudaErrors does,
+ * SRC/cuda_pointer.h:8-9; the C++ Gravity shim restores that behaviour).
+ *
+ * Threading: hh_set_jp is a pure host-memory write plus an atomic
+ * bookkeeping update, safe to call concurrently for distinct addr from
+ * OpenMP threads (the driver does so, SRC/nbodysystem.h:753-756).  Every
+ * other function must be called from one host thread at a time per context.
+ */
+#ifndef HERMITE_HIP_H
+#define HERMITE_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hh_ctx hh_ctx;
+
+/* Kernel variants. */
+enum {
+    HH_FP64 = 0,  /* all-FP64 interaction, FMA + FP32-seeded rsqrt refined to
+                     FP64; the drop-in for the TITAN back end */
+    HH_SPLIT = 1, /* FP64 relative quantities, FP32 interaction body, FP32
+                     per-tile partial sums folded into FP64 accumulators */
+    HH_EXACT = 2  /* IEEE FP64 with the generic back end's exact operation
+                     order (SRC/hermite{4,6,8}.h generic Gravity): results
+                     bit-identical to the reference CPU build */
+};
+
+/* Status codes. */
+enum {
+    HH_OK = 0,
+    HH_E_ARG = -1,     /* invalid argument (order, size, null pointer) */
+    HH_E_HIP = -2,     /* HIP runtime error */
+    HH_E_NODEV = -3,   /* no usable gfx950 device */
+    HH_E_STATE = -4    /* call out of sequence */
+};
+
+/* Doubles per record at the boundary and on the device. */
+int hh_particle_words(int order);  /* 24 / 28 / 36  (reference Particle)   */
+int hh_force_words(int order);     /* 6 / 9 / 12    (reference Force)      */
+int hh_jp_words(int order);        /* 14 / 20 / 26  (TITAN GParticle,
+                                      SRC/hermite6-titan.h:20-29)          */
+int hh_pred_words(int order);      /* 7 / 10 / 13   (TITAN GPredictor,
+                                      SRC/hermite6-titan.h:31-36)          */
+
+/* Gravity(int nbody): SRC/hermite6-titan.h:55-62.  order is 4, 6 or 8;
+ * variant one of HH_FP64, HH_SPLIT, HH_EXACT; device a HIP ordinal. */
+int hh_create(int order, int nbody, int variant, int device, hh_ctx **out);
+
+/* ~Gravity(): SRC/hermite6-titan.h:63-68. */
+int hh_destroy(hh_ctx *ctx);
+
+/* set_jp(addr, p): SRC/hermite6-titan.h:70-97.  Copies pos, mass, vel, tlast
+ * and the stored derivatives of one corrected Particle record into the
+ * pinned host j-store slot addr.  No HIP call. */
+int hh_set_jp(hh_ctx *ctx, int addr, const double *particle);
+
+/* Pointer to the pinned host j-store (nbody records of hh_jp_words doubles,
+ * word order pos[3] mass vel[3] tlast acc[3] jrk[3] snp[3] crk[3] d4a[3]
+ * d5a[3]); a caller that writes it directly must then call hh_mark_dirty. */
+double *hh_jstore(hh_ctx *ctx);
+int hh_mark_dirty(hh_ctx *ctx, int lo, int hi);
+
+/* predict_all(tsys): SRC/hermite6-titan.cu:269-283.  Uploads the slots
+ * written since the last upload, then predicts all nbody j-particles to
+ * tsys on the device. */
+int hh_predict_all(hh_ctx *ctx, double tsys);
+
+/* calc_force_on_first_nact(nact, eps2, force): SRC/hermite6-titan.cu:495-538.
+ * Forces (acc, jrk[, snp[, crk]]) on the predicted particles [0, nact) from
+ * all nbody predicted particles, written as nact Force records.  If pot is
+ * not NULL it also receives phi_i = -sum_{j != i} m_j / sqrt(r_ij^2 + eps2)
+ * over the predicted positions (an output the reference Force lacks). */
+int hh_force_first_nact(hh_ctx *ctx, int nact, double eps2, double *force, double *pot);
+
+/* calc_potential(eps2, potbuf): SRC/pot-titan.hu:131-171.  phi_i over the
+ * corrected (set_jp) positions of all nbody particles, self excluded. */
+int hh_potential(hh_ctx *ctx, double eps2, double *pot);
+
+/* Message of the last failure on ctx (or of the last failed hh_create when
+ * ctx is NULL). */
+const char *hh_last_error(const hh_ctx *ctx);
+
+/* ---- measurement and tuning (no reference counterpart) ---- */
+
+/* Force launch geometry.  kernel: 0 = automatic, 1 = i-parallel (i in lanes,
+ * j tiles in LDS), 2 = j-parallel (j in lanes, small i-groups);
+ * jsplit: j-slices per i-block (0 = automatic). */
+int hh_set_tuning(hh_ctx *ctx, int kernel, int jsplit);
+
+/* Runs the force on [0, nact) reps times on the device without copying the
+ * result back, and reports the mean device time per call in milliseconds,
+ * measured with HIP events on the context's stream. */
+int hh_time_force(hh_ctx *ctx, int nact, double eps2, int reps, double *ms_per_call);
+
+/* Same for predict_all (no upload) and the potential. */
+int hh_time_predict(hh_ctx *ctx, double tsys, int reps, double *ms_per_call);
+
+/* Launch the force on [0, nact) and return without waiting; the result
+ * stays in device memory until hh_synchronize / the next blocking call.
+ * For timing the device-resident hot path (predict + force). */
+int hh_force_async(hh_ctx *ctx, int nact, double eps2, int with_pot);
+int hh_predict_async(hh_ctx *ctx, double tsys);
+int hh_synchronize(hh_ctx *ctx);
+
+/* Geometry the last force call used: kernel id, j-slices, workgroups. */
+int hh_last_geometry(const hh_ctx *ctx, int *kernel, int *jsplit, int *workgroups);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HERMITE_HIP_H */
