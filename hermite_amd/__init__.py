@@ -56,6 +56,7 @@ def hip_lib():
         L.hh_last_error.argtypes = [vp]
         L.hh_last_error.restype = C.c_char_p
         L.hh_set_tuning.argtypes = [vp, C.c_int, C.c_int]
+        L.hh_set_tuning_ex.argtypes = [vp, C.c_int, C.c_int]
         L.hh_time_force.argtypes = [vp, C.c_int, C.c_double, C.c_int, _dp]
         L.hh_time_predict.argtypes = [vp, C.c_double, C.c_int, _dp]
         L.hh_force_async.argtypes = [vp, C.c_int, C.c_double, C.c_int]
@@ -75,6 +76,24 @@ def host_lib():
         L.hh_glibc_rand.argtypes = [C.c_uint, C.c_int, C.POINTER(C.c_int)]
         L.hh_plummer.argtypes = [C.c_int, C.c_uint, C.c_double, _dp, _dp, _dp]
         L.hh_salpeter.argtypes = [C.c_int, C.c_uint, C.c_double, C.c_double, C.c_double, _dp]
+        vp = C.c_void_p
+        L.hhd_create.restype = vp
+        L.hhd_create.argtypes = [C.c_int, C.c_int, _dp] + [C.c_double] * 6 + [C.c_int, C.c_int]
+        L.hhd_destroy.argtypes = [vp]
+        L.hhd_status.argtypes = [vp]
+        L.hhd_error.argtypes = [vp]
+        L.hhd_error.restype = C.c_char_p
+        for name in ("hhd_init_step", "hhd_integrate_one_block", "hhd_integrate_one_dtmax"):
+            getattr(L, name).argtypes = [vp]
+        L.hhd_integrate_until.argtypes = [vp, C.c_double]
+        for name in ("hhd_energy", "hhd_init_energy", "hhd_tsys"):
+            getattr(L, name).argtypes = [vp]
+            getattr(L, name).restype = C.c_double
+        for name in ("hhd_num_step", "hhd_num_bstep", "hhd_dtmin_hits"):
+            getattr(L, name).argtypes = [vp]
+            getattr(L, name).restype = C.c_long
+        L.hhd_get_ptcl.argtypes = [vp, _dp]
+        L.hhd_profile.argtypes = [vp, _dp]
         _host = L
     return _host
 
@@ -179,6 +198,9 @@ class Gravity:
     def set_tuning(self, kernel: int = 0, jsplit: int = 0):
         self._check(hip_lib().hh_set_tuning(self.h, kernel, jsplit), "hh_set_tuning")
 
+    def set_tuning_ex(self, i_per_thread: int = 0, split_fold: int = 0):
+        self._check(hip_lib().hh_set_tuning_ex(self.h, i_per_thread, split_fold), "hh_set_tuning_ex")
+
     def time_force(self, nact: int, eps2: float, reps: int = 10) -> float:
         ms = C.c_double()
         self._check(hip_lib().hh_time_force(self.h, nact, eps2, reps, C.byref(ms)), "hh_time_force")
@@ -202,3 +224,81 @@ class Gravity:
         k, s, w = C.c_int(), C.c_int(), C.c_int()
         hip_lib().hh_last_geometry(self.h, C.byref(k), C.byref(s), C.byref(w))
         return {"kernel": k.value, "jsplit": s.value, "workgroups": w.value}
+
+
+# ------------------------------------------------------------------ driver
+class System:
+    """Block-timestep driver (the reference's NbodySystem semantics,
+    SRC/nbodysystem.h) running on the HIP back end; native C++ in
+    libhermite_host.so."""
+
+    PHASES = ("force", "pot", "predict", "correct", "sort", "scan", "set_jp", "total")
+
+    def __init__(self, order, P, tsys, eps2, eta, eta_s, dtmax, dtmin=0.0, variant="fp64", device=0):
+        L = host_lib()
+        self.order, self.n = order, len(P)
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        self.h = L.hhd_create(order, self.n, _p(P), tsys, eps2, eta, eta_s, dtmax, dtmin, VARIANTS[variant], device)
+        if not self.h:
+            raise HHError("hhd_create: bad arguments")
+        self._check(L.hhd_status(self.h), "hhd_create")
+
+    def _check(self, rc, what):
+        if rc < 0:
+            raise HHError(f"{what} failed ({rc}): {host_lib().hhd_error(self.h).decode()}")
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            host_lib().hhd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def init_step(self):
+        self._check(host_lib().hhd_init_step(self.h), "init_step")
+
+    def integrate_one_block(self) -> int:
+        return self._check(host_lib().hhd_integrate_one_block(self.h), "integrate_one_block")
+
+    def integrate_one_dtmax(self):
+        self._check(host_lib().hhd_integrate_one_dtmax(self.h), "integrate_one_dtmax")
+
+    def integrate_until(self, tend):
+        self._check(host_lib().hhd_integrate_until(self.h, tend), "integrate_until")
+
+    def energy(self) -> float:
+        e = host_lib().hhd_energy(self.h)
+        self._check(host_lib().hhd_status(self.h), "energy")
+        return e
+
+    @property
+    def init_energy(self):
+        return host_lib().hhd_init_energy(self.h)
+
+    @property
+    def tsys(self):
+        return host_lib().hhd_tsys(self.h)
+
+    @property
+    def num_step(self):
+        return host_lib().hhd_num_step(self.h)
+
+    @property
+    def num_bstep(self):
+        return host_lib().hhd_num_bstep(self.h)
+
+    @property
+    def dtmin_hits(self):
+        return host_lib().hhd_dtmin_hits(self.h)
+
+    def ptcl(self) -> np.ndarray:
+        out = np.zeros((self.n, PSTRIDE[self.order]))
+        host_lib().hhd_get_ptcl(self.h, _p(out))
+        return out
+
+    def profile(self) -> dict:
+        out = np.zeros(len(self.PHASES))
+        host_lib().hhd_profile(self.h, _p(out))
+        return dict(zip(self.PHASES, out.tolist()))

@@ -1,0 +1,550 @@
+// Block-timestep driver on the gfx950 back end.
+//
+// The caller side of the force + predictor path: the reference's
+// NbodySystem (SRC/nbodysystem.h:253-970, TITAN flow without FAST_OMP_SYNC)
+// with its per-order Particle physics (SRC/hermite{4,6,8}.h: corrector,
+// derivative reconstruction, quantised Aarseth step), driving the HIP
+// back end through the C-ABI of include/hermite_hip.h.  Floating-point
+// expressions keep the reference's operation order and this file is
+// compiled without FMA contraction, so with the HH_EXACT variant a run is
+// bit-identical to the reference's CPU build.
+//
+// Exposed to tests, the benchmark and hermite_run through extern "C" hhd_*.
+#include "hermite_hip.h"
+
+#include <omp.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace {
+
+// 3-vector with the operator semantics of the reference's dvec3
+// (SRC/vector3.h:105-150): a*b is the dot product ((x*x' + y*y') + z*z'),
+// v/s multiplies by the rounded reciprocal 1/s.
+struct V3 {
+    double x, y, z;
+    V3() : x(0), y(0), z(0) {}
+    V3(double a, double b, double c) : x(a), y(b), z(c) {}
+    explicit V3(const double *p) : x(p[0]), y(p[1]), z(p[2]) {}
+    void store(double *p) const { p[0] = x; p[1] = y; p[2] = z; }
+    V3 operator+(const V3 &o) const { return V3(x + o.x, y + o.y, z + o.z); }
+    V3 operator-(const V3 &o) const { return V3(x - o.x, y - o.y, z - o.z); }
+    V3 operator-() const { return V3(-x, -y, -z); }
+    V3 operator*(double s) const { return V3(x * s, y * s, z * s); }
+    V3 operator/(double s) const { const double r = 1.0 / s; return *this * r; }
+    double operator*(const V3 &o) const { return x * o.x + y * o.y + z * o.z; }
+    double norm2() const { return (*this) * (*this); }
+};
+inline V3 operator*(double s, const V3 &v) { return v * s; }
+
+// Particle record words (the reference Particle, SRC/hermite8.h:31-45)
+enum { ID = 0, MASS = 1, TLAST = 2, DT = 3, POS = 4, VEL = 7, ACC = 10, JRK = 13, SNP = 16, CRK = 19,
+       D4A = 22, D5A = 25, D6A = 28, D7A = 31 };
+constexpr int pwords(int o) { return o == 4 ? 24 : o == 6 ? 28 : 36; }
+constexpr int fwords(int o) { return o == 4 ? 6 : o == 6 ? 9 : 12; }
+
+// pow_one_nth_quant<N>: 2^floor(log2(x)/N) from the exponent bits
+// (SRC/pownth.h:3-19).
+template <int N>
+double pow_nth_quant(double x)
+{
+    uint64_t b;
+    std::memcpy(&b, &x, 8);
+    uint64_t e = b >> 52;
+    e = (e - (uint64_t)(1023 % N)) / N + (uint64_t)(1023 - 1023 / N);
+    b = e << 52;
+    std::memcpy(&x, &b, 8);
+    return x;
+}
+
+// h = dt/2 and 1/h for a power-of-two dt by exponent arithmetic
+// (SRC/pownth.h:21-34).
+inline void half_and_inverse(double dt, double &h, double &hinv)
+{
+    uint64_t b;
+    std::memcpy(&b, &dt, 8);
+    b -= (uint64_t)1 << 52;
+    std::memcpy(&h, &b, 8);
+    b = ((uint64_t)2046 << 52) - b;
+    std::memcpy(&hinv, &b, 8);
+}
+
+template <int ORDER>
+struct Phys;
+
+// 4th order: SRC/hermite4.h:67-153
+template <>
+struct Phys<4> {
+    static void assign(double *p, const double *f) { V3(f).store(p + ACC); V3(f + 3).store(p + JRK); }
+    static void init_dt(double *p, double eta_s, double dtmax)
+    {
+        const double dtnat = eta_s * std::sqrt(V3(p + ACC).norm2() / V3(p + JRK).norm2());
+        double dt = 0.25 * dtmax;
+        while (dt > dtnat) dt *= 0.5;
+        p[DT] = dt;
+    }
+    static double step(const double *p, double etapow)
+    {
+        const double s0 = V3(p + ACC).norm2(), s1 = V3(p + JRK).norm2();
+        const double s2 = V3(p + SNP).norm2(), s3 = V3(p + CRK).norm2();
+        const double u = std::sqrt(s0 * s2) + s1, l = std::sqrt(s1 * s3) + s2;
+        return pow_nth_quant<2>(etapow * (u / l));
+    }
+    static void correct(double *p, const double *f, double etapow, double dtlim)
+    {
+        double h, hinv;
+        half_and_inverse(p[DT], h, hinv);
+        const V3 fa(f), fj(f + 3), acc(p + ACC), jrk(p + JRK), vel(p + VEL);
+        const V3 Ap = fa + acc, Am = fa - acc;
+        const V3 Jp = (fj + jrk) * h, Jm = (fj - jrk) * h;
+        const V3 vel1 = vel + h * (Ap - (1. / 3.) * Jm);
+        const V3 pos = V3(p + POS) + h * ((vel + vel1) + h * (-(1. / 3.) * Am));
+        pos.store(p + POS);
+        vel1.store(p + VEL);
+        fa.store(p + ACC);
+        fj.store(p + JRK);
+        p[TLAST] += p[DT];
+        V3 snp = (0.5 * hinv * hinv) * Jm;
+        const V3 crk = (1.5 * hinv * hinv * hinv) * (Jp - Am);
+        snp = snp + h * crk;
+        snp.store(p + SNP);
+        crk.store(p + CRK);
+        const double dtq = step(p, etapow);
+        p[DT] = dtq < dtlim ? dtq : dtlim;
+    }
+};
+
+// 6th order: SRC/hermite6.h:75-184
+template <>
+struct Phys<6> {
+    static void assign(double *p, const double *f)
+    {
+        V3(f).store(p + ACC); V3(f + 3).store(p + JRK); V3(f + 6).store(p + SNP);
+    }
+    static void init_dt(double *p, double eta_s, double dtmax)
+    {
+        const double dtnat = eta_s * std::sqrt(std::sqrt(V3(p + ACC).norm2() / V3(p + SNP).norm2()));
+        double dt = 0.25 * dtmax;
+        while (dt > dtnat) dt *= 0.5;
+        p[DT] = dt;
+    }
+    static double step(const double *p, double etapow)
+    {
+        const double s0 = V3(p + ACC).norm2(), s1 = V3(p + JRK).norm2(), s2 = V3(p + SNP).norm2();
+        const double s3 = V3(p + CRK).norm2(), s4 = V3(p + D4A).norm2(), s5 = V3(p + D5A).norm2();
+        const double u = std::sqrt(s0 * s2) + s1, l = std::sqrt(s3 * s5) + s4;
+        return pow_nth_quant<6>(etapow * (u / l));
+    }
+    static void correct(double *p, const double *f, double etapow, double dtlim)
+    {
+        double h, hinv;
+        half_and_inverse(p[DT], h, hinv);
+        const V3 fa(f), fj(f + 3), fs(f + 6), acc(p + ACC), jrk(p + JRK), snp(p + SNP), vel(p + VEL);
+        const V3 Ap = fa + acc, Am = fa - acc;
+        const V3 Jp = (fj + jrk) * h, Jm = (fj - jrk) * h;
+        const V3 Sp = (fs + snp) * (h * h), Sm = (fs - snp) * (h * h);
+        const V3 vel1 = vel + h * (Ap - 0.4 * Jm + (1. / 15) * Sp);
+        const V3 pos = V3(p + POS) + h * ((vel + vel1) + h * (-0.4 * Am + (1. / 15) * Jp));
+        pos.store(p + POS);
+        vel1.store(p + VEL);
+        fa.store(p + ACC);
+        fj.store(p + JRK);
+        fs.store(p + SNP);
+        p[TLAST] += p[DT];
+        const double hinv2 = hinv * hinv, hinv3 = hinv2 * hinv, hinv4 = hinv2 * hinv2, hinv5 = hinv2 * hinv3;
+        V3 crk = (0.75 * hinv3) * (-5. * Am + 5. * Jp - Sm);
+        V3 d4a = (1.5 * hinv4) * (-Jm + Sp);
+        const V3 d5a = (7.5 * hinv5) * (3 * Am - 3 * Jp + Sm);
+        crk = crk + h * (d4a + 0.5 * h * d5a);
+        d4a = d4a + h * d5a;
+        crk.store(p + CRK);
+        d4a.store(p + D4A);
+        d5a.store(p + D5A);
+        const double dtq = step(p, etapow);
+        p[DT] = dtq < dtlim ? dtq : dtlim;
+    }
+};
+
+// 8th order: SRC/hermite8.h:91-222
+template <>
+struct Phys<8> {
+    static void assign(double *p, const double *f)
+    {
+        V3(f).store(p + ACC); V3(f + 3).store(p + JRK); V3(f + 6).store(p + SNP); V3(f + 9).store(p + CRK);
+    }
+    static void init_dt(double *p, double eta_s, double dtmax)
+    {
+        const double s0 = V3(p + ACC).norm2(), s1 = V3(p + JRK).norm2();
+        const double s2 = V3(p + SNP).norm2(), s3 = V3(p + CRK).norm2();
+        const double u = std::sqrt(s0 * s2) + s1, l = std::sqrt(s1 * s3) + s2;
+        const double dtnat = eta_s * std::sqrt(u / l);
+        double dt = 0.25 * dtmax;
+        while (dt > dtnat) dt *= 0.5;
+        p[DT] = dt;
+    }
+    static double step(const double *p, double etapow)
+    {
+        const double s0 = V3(p + ACC).norm2(), s1 = V3(p + JRK).norm2(), s2 = V3(p + SNP).norm2();
+        const double s5 = V3(p + D5A).norm2(), s6 = V3(p + D6A).norm2(), s7 = V3(p + D7A).norm2();
+        const double u = std::sqrt(s0 * s2) + s1, l = std::sqrt(s5 * s7) + s6;
+        return pow_nth_quant<10>(etapow * (u / l));
+    }
+    static void correct(double *p, const double *f, double etapow, double dtlim)
+    {
+        double h, hinv;
+        half_and_inverse(p[DT], h, hinv);
+        const V3 fa(f), fj(f + 3), fs(f + 6), fc(f + 9);
+        const V3 acc(p + ACC), jrk(p + JRK), snp(p + SNP), crk(p + CRK), vel(p + VEL);
+        const V3 Ap = fa + acc, Am = fa - acc;
+        const V3 Jp = (fj + jrk) * h, Jm = (fj - jrk) * h;
+        const V3 Sp = (fs + snp) * (h * h), Sm = (fs - snp) * (h * h);
+        const V3 Cp = (fc + crk) * (h * h * h), Cm = (fc - crk) * (h * h * h);
+        const V3 vel1 = vel + h * (Ap + (-3. / 7.) * Jm + (2. / 21) * Sp - (1. / 105.) * Cm);
+        const V3 pos = V3(p + POS) + h * ((vel + vel1) + h * ((-3. / 7.) * Am + (2. / 21) * Jp - (1. / 105.) * Sm));
+        pos.store(p + POS);
+        vel1.store(p + VEL);
+        fa.store(p + ACC);
+        fj.store(p + JRK);
+        fs.store(p + SNP);
+        fc.store(p + CRK);
+        p[TLAST] += p[DT];
+        const double hinv2 = hinv * hinv, hinv3 = hinv2 * hinv, hinv4 = hinv2 * hinv2;
+        const double hinv5 = hinv2 * hinv3, hinv6 = hinv3 * hinv3, hinv7 = hinv3 * hinv4;
+        V3 d4a = (hinv4 * 24. / 32.) * (-5. * Jm + 5. * Sp - Cm);
+        V3 d5a = (hinv5 * 120. / 32.) * (21. * Am - 21. * Jp + 8. * Sm - Cp);
+        V3 d6a = (hinv6 * 720. / 32.) * (Jm - Sp + Cm / 3.);
+        const V3 d7a = (hinv7 * 5040. / 32.) * (-5. * Am + 5. * Jp - 2. * Sm + Cp / 3.);
+        const double h2 = 0.5 * h, h3 = (1. / 3.) * h;
+        d4a = d4a + h * (d5a + h2 * (d6a + h3 * d7a));
+        d5a = d5a + h * (d6a + h2 * d7a);
+        d6a = d6a + h * d7a;
+        d4a.store(p + D4A);
+        d5a.store(p + D5A);
+        d6a.store(p + D6A);
+        d7a.store(p + D7A);
+        const double dtq = step(p, etapow);
+        p[DT] = dtq < dtlim ? dtq : dtlim;
+    }
+};
+
+enum Phase { P_FORCE = 0, P_POT, P_PREDICT, P_CORRECT, P_SORT, P_SCAN, P_SETJP, P_TOTAL, P_NUM };
+
+}  // namespace
+
+struct hhd_sys {
+    int order = 0, n = 0, ps = 0, fs = 0;
+    double eps2 = 0, eta = 0, etapow = 0, eta_s = 0, dtmax = 0, dtmin = 0, tsys = 0;
+    double init_energy = 0, prev_energy = 0;
+    long num_step = 0, num_bstep = 0, dtmin_hits = 0;
+    std::vector<double> ptcl, force, pot, dtbuf, tmp;
+    hh_ctx *g = nullptr;
+    double prof[P_NUM] = {0};
+    std::string err;
+    int rc = HH_OK;
+
+    double *P(int i) { return ptcl.data() + (size_t)i * ps; }
+
+    bool ok(int r, const char *what)
+    {
+        if (r != HH_OK && rc == HH_OK) {
+            rc = r;
+            err = std::string(what) + ": " + hh_last_error(g);
+        }
+        return r == HH_OK;
+    }
+
+    void set_jp(int i) { ok(hh_set_jp(g, i, P(i)), "hh_set_jp"); }
+
+    void assign(int i)
+    {
+        const double *f = force.data() + (size_t)i * fs;
+        switch (order) {
+        case 4: Phys<4>::assign(P(i), f); break;
+        case 6: Phys<6>::assign(P(i), f); break;
+        default: Phys<8>::assign(P(i), f); break;
+        }
+    }
+    void init_dt(int i)
+    {
+        switch (order) {
+        case 4: Phys<4>::init_dt(P(i), eta_s, dtmax); break;
+        case 6: Phys<6>::init_dt(P(i), eta_s, dtmax); break;
+        default: Phys<8>::init_dt(P(i), eta_s, dtmax); break;
+        }
+    }
+    void correct(int i, double dtlim)
+    {
+        const double *f = force.data() + (size_t)i * fs;
+        switch (order) {
+        case 4: Phys<4>::correct(P(i), f, etapow, dtlim); break;
+        case 6: Phys<6>::correct(P(i), f, etapow, dtlim); break;
+        default: Phys<8>::correct(P(i), f, etapow, dtlim); break;
+        }
+    }
+
+    void swap(int i, int j)
+    {
+        std::memcpy(tmp.data(), P(i), 8 * ps);
+        std::memcpy(P(i), P(j), 8 * ps);
+        std::memcpy(P(j), tmp.data(), 8 * ps);
+    }
+
+    // sort_ptcl(nact, dtlim): groups by dt level with the two-ended swap
+    // partition std::partition uses (SRC/nbodysystem.h:486-504).
+    void sort_by_level(int nact, double dtlim)
+    {
+        int end = nact;
+        double dt = dtlim;
+        while (end != 0) {
+            int first = 0, last = end;
+            for (;;) {
+                while (first != last && P(first)[DT] < dt) ++first;
+                if (first == last) break;
+                --last;
+                while (first != last && !(P(last)[DT] < dt)) --last;
+                if (first == last) break;
+                swap(first, last);
+                ++first;
+            }
+            end = first;
+            dt *= 0.5;
+        }
+        for (int i = 0; i < nact; i++) dtbuf[i] = P(i)[DT] + P(i)[TLAST];
+    }
+
+    // sort_ptcl_dtcache: the same partition keyed on the dt bit pattern,
+    // halving by decrementing the exponent field (SRC/nbodysystem.h:505-555).
+    void sort_by_level_cached(int nact, double dtlim)
+    {
+        int64_t key;
+        std::memcpy(&key, &dtlim, 8);
+        int end = nact;
+        auto bits = [this](int i) { int64_t b; std::memcpy(&b, &dtbuf[i], 8); return b; };
+        while (end != 0) {
+            int i = 0, j = end;
+            for (;;) {
+                while (i != j && bits(i) != key) ++i;
+                if (i == j) break;
+                --j;
+                while (i != j && bits(j) == key) --j;
+                if (i == j) break;
+                std::swap(dtbuf[i], dtbuf[j]);
+                swap(i, j);
+                ++i;
+            }
+            end = i;
+            key -= (int64_t)1 << 52;
+        }
+        for (int i = 0; i < nact; i++) dtbuf[i] = P(i)[DT] + P(i)[TLAST];
+    }
+
+    double calc_dtlim(double tnext) const
+    {
+        double dtlim = dtmax, s = tnext / dtmax;
+        while (s != std::floor(s)) {
+            s *= 2.0;
+            dtlim *= 0.5;
+        }
+        return dtlim;
+    }
+
+    int count_nact(double tnext) const
+    {
+        int k = 0;
+        while (k < n && dtbuf[k] == tnext) k++;
+        return k;
+    }
+
+    double energy()
+    {
+        const double t0 = omp_get_wtime();
+        ok(hh_potential(g, eps2, pot.data()), "hh_potential");
+        prof[P_POT] += omp_get_wtime() - t0;
+        double ke = 0.0, pe = 0.0;
+        for (int i = 0; i < n; i++) {
+            ke += P(i)[MASS] * V3(P(i) + VEL).norm2();
+            pe += P(i)[MASS] * pot[i];
+        }
+        ke *= 0.5;
+        pe *= 0.5;
+        return ke + pe;
+    }
+
+    // init_step: SRC/nbodysystem.h:565-631
+    void init_step()
+    {
+        for (int i = 0; i < n; i++) {
+            P(i)[TLAST] = tsys;
+            set_jp(i);
+        }
+        init_energy = prev_energy = energy();
+        if (order > 4) {
+            ok(hh_predict_all(g, tsys), "hh_predict_all");
+            ok(hh_force_first_nact(g, n, eps2, force.data(), nullptr), "hh_force_first_nact");
+            for (int i = 0; i < n; i++) {
+                assign(i);
+                set_jp(i);
+            }
+        }
+        double t0 = omp_get_wtime();
+        ok(hh_predict_all(g, tsys), "hh_predict_all");
+        double t1 = omp_get_wtime();
+        ok(hh_force_first_nact(g, n, eps2, force.data(), nullptr), "hh_force_first_nact");
+        double t2 = omp_get_wtime();
+        prof[P_PREDICT] += t1 - t0;
+        prof[P_FORCE] += t2 - t1;
+        for (int i = 0; i < n; i++) {
+            assign(i);
+            init_dt(i);
+        }
+        sort_by_level(n, dtmax);
+        for (int i = 0; i < n; i++) set_jp(i);
+        init_energy = energy();
+    }
+
+    // integrate_one_block: SRC/nbodysystem.h:692-765.  Returns nact.
+    int integrate_one_block()
+    {
+        double t0 = omp_get_wtime();
+        const double tnext = P(0)[TLAST] + P(0)[DT];
+        const double dtlim = calc_dtlim(tnext);
+        const int nact = count_nact(tnext);
+        double t1 = omp_get_wtime();
+        ok(hh_predict_all(g, tnext), "hh_predict_all");
+        double t2 = omp_get_wtime();
+        ok(hh_force_first_nact(g, nact, eps2, force.data(), nullptr), "hh_force_first_nact");
+        double t3 = omp_get_wtime();
+        long hits = 0;
+#pragma omp parallel for reduction(+ : hits) if (nact > 256)
+        for (int i = 0; i < nact; i++) {
+            correct(i, dtlim);
+            if (dtmin > 0.0 && P(i)[DT] < dtmin) {
+                P(i)[DT] = dtmin;
+                hits++;
+            }
+            dtbuf[i] = P(i)[DT];
+        }
+        dtmin_hits += hits;
+        double t4 = omp_get_wtime();
+        sort_by_level_cached(nact, dtlim);
+        double t5 = omp_get_wtime();
+#pragma omp parallel for if (nact > 256)
+        for (int i = 0; i < nact; i++) set_jp(i);
+        double t6 = omp_get_wtime();
+        prof[P_SCAN] += t1 - t0;
+        prof[P_PREDICT] += t2 - t1;
+        prof[P_FORCE] += t3 - t2;
+        prof[P_CORRECT] += t4 - t3;
+        prof[P_SORT] += t5 - t4;
+        prof[P_SETJP] += t6 - t5;
+        num_step += nact;
+        num_bstep++;
+        tsys = tnext;
+        return nact;
+    }
+
+    void integrate_one_dtmax()
+    {
+        const double t0 = omp_get_wtime();
+        const double tt = tsys + dtmax;
+        while (tsys < tt && rc == HH_OK) integrate_one_block();
+        prof[P_TOTAL] += omp_get_wtime() - t0;
+    }
+};
+
+extern "C" {
+
+// Creates the driver around a new back-end context.  recs: n reference
+// Particle records (id, mass, pos, vel; other words ignored).
+hhd_sys *hhd_create(int order, int n, const double *recs, double tsys, double eps2, double eta, double eta_s,
+                    double dtmax, double dtmin, int variant, int device)
+{
+    if ((order != 4 && order != 6 && order != 8) || n <= 0 || !recs) return nullptr;
+    hhd_sys *s = new hhd_sys();
+    s->order = order;
+    s->n = n;
+    s->ps = pwords(order);
+    s->fs = fwords(order);
+    s->eps2 = eps2;
+    s->eta = eta;
+    s->eta_s = eta_s;
+    s->dtmax = dtmax;
+    s->dtmin = dtmin;
+    s->tsys = tsys;
+    s->etapow = std::pow(eta, double(2 * (order - 3)));
+    s->ptcl.assign(recs, recs + (size_t)n * s->ps);
+    s->force.assign((size_t)n * s->fs, 0.0);
+    s->pot.assign(n, 0.0);
+    s->dtbuf.assign(n, 0.0);
+    s->tmp.assign(s->ps, 0.0);
+    if (hh_create(order, n, variant, device, &s->g) != HH_OK) {
+        s->rc = HH_E_NODEV;
+        s->err = std::string("hh_create: ") + hh_last_error(nullptr);
+        s->g = nullptr;
+    }
+    return s;
+}
+
+void hhd_destroy(hhd_sys *s)
+{
+    if (!s) return;
+    if (s->g) hh_destroy(s->g);
+    delete s;
+}
+
+int hhd_status(const hhd_sys *s) { return s ? s->rc : HH_E_ARG; }
+const char *hhd_error(const hhd_sys *s) { return s ? s->err.c_str() : "null driver"; }
+
+int hhd_init_step(hhd_sys *s)
+{
+    if (!s || !s->g) return HH_E_STATE;
+    s->init_step();
+    return s->rc;
+}
+
+int hhd_integrate_one_block(hhd_sys *s)
+{
+    if (!s || !s->g) return HH_E_STATE;
+    const int nact = s->integrate_one_block();
+    return s->rc == HH_OK ? nact : s->rc;
+}
+
+int hhd_integrate_one_dtmax(hhd_sys *s)
+{
+    if (!s || !s->g) return HH_E_STATE;
+    s->integrate_one_dtmax();
+    return s->rc;
+}
+
+int hhd_integrate_until(hhd_sys *s, double tend)
+{
+    if (!s || !s->g) return HH_E_STATE;
+    while (s->tsys < tend && s->rc == HH_OK) s->integrate_one_dtmax();
+    return s->rc;
+}
+
+double hhd_energy(hhd_sys *s) { return s && s->g ? s->energy() : NAN; }
+double hhd_init_energy(const hhd_sys *s) { return s ? s->init_energy : NAN; }
+double hhd_tsys(const hhd_sys *s) { return s ? s->tsys : NAN; }
+long hhd_num_step(const hhd_sys *s) { return s ? s->num_step : -1; }
+long hhd_num_bstep(const hhd_sys *s) { return s ? s->num_bstep : -1; }
+long hhd_dtmin_hits(const hhd_sys *s) { return s ? s->dtmin_hits : -1; }
+
+void hhd_get_ptcl(const hhd_sys *s, double *out)
+{
+    if (s) std::memcpy(out, s->ptcl.data(), sizeof(double) * s->ptcl.size());
+}
+
+// Wall seconds per phase: force, pot, predict, correct, sort, scan, set_jp, total.
+void hhd_profile(const hhd_sys *s, double *out)
+{
+    if (s) std::memcpy(out, s->prof, sizeof(s->prof));
+}
+
+}  // extern "C"

@@ -35,7 +35,7 @@ struct hh_ctx {
     unsigned *d_cnt = nullptr;  // per-i-block arrival counters
     int cnt_cap = 0;
     std::atomic<int> dirty_hi{0};
-    int tune_kernel = 0, tune_split = 0;
+    int tune_kernel = 0, tune_split = 0, tune_ni = 0, tune_fold = 0;
     int last_kernel = 0, last_split = 0, last_wgs = 0;
     std::string err;
 };
@@ -149,12 +149,14 @@ static int ensure_part(hh_ctx *c, size_t need)
 
 // Workgroups wanted per force launch: several per CU on 256 CUs.
 static constexpr int TARGET_WG = 2048;
+// Up to this many j-slices the last workgroup of each i-block adds them;
+// beyond it a separate slice_reduce_kernel does, with the whole GPU.
+static constexpr int LAST_BLOCK_MAX = 16;
 
 // i-parallel launch of tile_kernel over src slots [0, ni) as i-particles.
-template <int ORDER, int MODE, bool POT>
-static int launch_tile(hh_ctx *c, const double *src, int ni, double eps2, double *fout, double *pout)
+template <int ORDER, int MODE, bool POT, int NI, int FOLD>
+static int launch_tile_ni(hh_ctx *c, const double *src, int ni, double eps2, double *fout, double *pout)
 {
-    constexpr int NI = 1;
     constexpr int NC = n_acc(ORDER, POT);
     const int bi = NT * NI;
     const int iblocks = (ni + bi - 1) / bi;
@@ -179,6 +181,7 @@ static int launch_tile(hh_ctx *c, const double *src, int ni, double eps2, double
     a.pot_mask_r2 = (ORDER == 0 && c->order == 4) ? 1 : 0;
     a.part = nullptr;
     a.cnt = c->d_cnt;
+    a.last_block = nslice <= LAST_BLOCK_MAX;
     a.fout = fout;
     a.pout = pout;
     if (nslice > 1) {
@@ -187,8 +190,14 @@ static int launch_tile(hh_ctx *c, const double *src, int ni, double eps2, double
         if (r) return r;
         a.part = c->d_part;
     }
-    hipLaunchKernelGGL((tile_kernel<ORDER, MODE, NI, POT>), dim3(iblocks, nslice), dim3(NT), 0, c->stream, a);
-    const hipError_t e = hipGetLastError();
+    hipLaunchKernelGGL((tile_kernel<ORDER, MODE, NI, POT, FOLD>), dim3(iblocks, nslice), dim3(NT), 0, c->stream, a);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && nslice > 1 && !a.last_block) {
+        const int nt = ni * NC;
+        hipLaunchKernelGGL(slice_reduce_kernel, dim3((nt + 255) / 256), dim3(256), 0, c->stream,
+                           (const double *)a.part, nslice, NC, ORDER == 0 ? 1 : f_words(ORDER), a.ipad, ni, fout, pout);
+        e = hipGetLastError();
+    }
     if (e != hipSuccess) return fail(c, HH_E_HIP, std::string("force launch: ") + hipGetErrorString(e));
     c->last_kernel = 1;
     c->last_split = nslice;
@@ -196,21 +205,107 @@ static int launch_tile(hh_ctx *c, const double *src, int ni, double eps2, double
     return HH_OK;
 }
 
+// Default FP32 partial-sum length of the SPLIT body.
+static constexpr int DEFAULT_FOLD = 8;
+
+template <int ORDER, int MODE, bool POT>
+static int launch_tile(hh_ctx *c, const double *src, int ni, double eps2, double *fout, double *pout)
+{
+    if constexpr (MODE == EXACT) {
+        return launch_tile_ni<ORDER, MODE, POT, 1, 1>(c, src, ni, eps2, fout, pout);
+    } else if constexpr (MODE == FP64) {
+        if (c->tune_ni == 2) return launch_tile_ni<ORDER, MODE, POT, 2, 1>(c, src, ni, eps2, fout, pout);
+        return launch_tile_ni<ORDER, MODE, POT, 1, 1>(c, src, ni, eps2, fout, pout);
+    } else {
+        const int fold = c->tune_fold ? c->tune_fold : DEFAULT_FOLD;
+        if (c->tune_ni == 2) {
+            switch (fold) {
+            case 4: return launch_tile_ni<ORDER, MODE, POT, 2, 4>(c, src, ni, eps2, fout, pout);
+            case 16: return launch_tile_ni<ORDER, MODE, POT, 2, 16>(c, src, ni, eps2, fout, pout);
+            case 32: return launch_tile_ni<ORDER, MODE, POT, 2, 32>(c, src, ni, eps2, fout, pout);
+            default: return launch_tile_ni<ORDER, MODE, POT, 2, 8>(c, src, ni, eps2, fout, pout);
+            }
+        }
+        switch (fold) {
+        case 4: return launch_tile_ni<ORDER, MODE, POT, 1, 4>(c, src, ni, eps2, fout, pout);
+        case 16: return launch_tile_ni<ORDER, MODE, POT, 1, 16>(c, src, ni, eps2, fout, pout);
+        case 32: return launch_tile_ni<ORDER, MODE, POT, 1, 32>(c, src, ni, eps2, fout, pout);
+        default: return launch_tile_ni<ORDER, MODE, POT, 1, 8>(c, src, ni, eps2, fout, pout);
+        }
+    }
+}
+
+// Largest active count routed to the j-parallel kernel by default.
+static constexpr int JPAR_MAX = 64;
+static constexpr int TARGET_WG_JPAR = 1024;
+
+// j-parallel launch (jpar_kernel) over [0, nact) as i-particles.
+template <int ORDER, int MODE, int GI, bool POT>
+static int launch_jpar(hh_ctx *c, int nact, double eps2)
+{
+    constexpr int V = GI * n_acc(ORDER, POT);
+    const int ngroups = (nact + GI - 1) / GI;
+    const int ntj = c->npad / NT;
+    int nslice = c->tune_split > 0 ? c->tune_split : (TARGET_WG_JPAR + ngroups - 1) / ngroups;
+    nslice = std::max(1, std::min(nslice, ntj));
+    const int jchunk = (ntj + nslice - 1) / nslice * NT;
+    nslice = (c->npad + jchunk - 1) / jchunk;
+    JParArgs a;
+    a.src = c->d_jpr;
+    a.npad = c->npad;
+    a.nact = nact;
+    a.ngroups = ngroups;
+    a.jchunk = jchunk;
+    a.nslice = nslice;
+    a.eps2 = eps2;
+    a.part = nullptr;
+    a.cnt = c->d_cnt;
+    a.fout = c->d_force;
+    a.pout = c->d_pot;
+    if (nslice > 1) {
+        if (ngroups > c->cnt_cap) return fail(c, HH_E_STATE, "counter buffer too small");
+        const int r = ensure_part(c, (size_t)nslice * ngroups * V);
+        if (r) return r;
+        a.part = c->d_part;
+    }
+    hipLaunchKernelGGL((jpar_kernel<ORDER, MODE, GI, POT>), dim3(ngroups, nslice), dim3(NT), 0, c->stream, a);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(c, HH_E_HIP, std::string("force launch: ") + hipGetErrorString(e));
+    c->last_kernel = 2;
+    c->last_split = nslice;
+    c->last_wgs = ngroups * nslice;
+    return HH_OK;
+}
+
+template <int ORDER, int MODE, bool POT>
+static int launch_jpar_gi(hh_ctx *c, int nact, double eps2)
+{
+    if (nact == 1) return launch_jpar<ORDER, MODE, 1, POT>(c, nact, eps2);
+    if (nact == 2) return launch_jpar<ORDER, MODE, 2, POT>(c, nact, eps2);
+    return launch_jpar<ORDER, MODE, 4, POT>(c, nact, eps2);
+}
+
+template <int ORDER, int MODE, bool POT>
+static int launch_force_mode(hh_ctx *c, int nact, double eps2)
+{
+    int kernel = c->tune_kernel;
+    if (kernel == 0) kernel = nact <= JPAR_MAX ? 2 : 1;
+    if constexpr (MODE != EXACT) {
+        if (kernel == 2) return launch_jpar_gi<ORDER, MODE, POT>(c, nact, eps2);
+    }
+    return launch_tile<ORDER, MODE, POT>(c, c->d_jpr, nact, eps2, c->d_force, c->d_pot);
+}
+
 template <int ORDER>
 static int launch_force_order(hh_ctx *c, int nact, double eps2, bool pot)
 {
-    double *fo = c->d_force, *po = c->d_pot;
-    const double *src = c->d_jpr;
     switch (c->variant) {
     case HH_FP64:
-        return pot ? launch_tile<ORDER, FP64, true>(c, src, nact, eps2, fo, po)
-                   : launch_tile<ORDER, FP64, false>(c, src, nact, eps2, fo, po);
+        return pot ? launch_force_mode<ORDER, FP64, true>(c, nact, eps2) : launch_force_mode<ORDER, FP64, false>(c, nact, eps2);
     case HH_SPLIT:
-        return pot ? launch_tile<ORDER, SPLIT, true>(c, src, nact, eps2, fo, po)
-                   : launch_tile<ORDER, SPLIT, false>(c, src, nact, eps2, fo, po);
+        return pot ? launch_force_mode<ORDER, SPLIT, true>(c, nact, eps2) : launch_force_mode<ORDER, SPLIT, false>(c, nact, eps2);
     default:
-        return pot ? launch_tile<ORDER, EXACT, true>(c, src, nact, eps2, fo, po)
-                   : launch_tile<ORDER, EXACT, false>(c, src, nact, eps2, fo, po);
+        return pot ? launch_force_mode<ORDER, EXACT, true>(c, nact, eps2) : launch_force_mode<ORDER, EXACT, false>(c, nact, eps2);
     }
 }
 
@@ -285,7 +380,7 @@ int hh_create(int order, int nbody, int variant, int device, hh_ctx **out)
     if ((e = hipMalloc(&c->d_jpr, sizeof(double) * Q * np)) != hipSuccess) return bail("predictions", e);
     if ((e = hipMalloc(&c->d_force, sizeof(double) * F * nbody)) != hipSuccess) return bail("force", e);
     if ((e = hipMalloc(&c->d_pot, sizeof(double) * np)) != hipSuccess) return bail("pot", e);
-    c->cnt_cap = (int)(np / NT) + 1;
+    c->cnt_cap = (int)np;  // one per i-group of the finest grouping
     if ((e = hipMalloc(&c->d_cnt, sizeof(unsigned) * c->cnt_cap)) != hipSuccess) return bail("counters", e);
     if ((e = hipMemsetAsync(c->d_cnt, 0, sizeof(unsigned) * c->cnt_cap, c->stream)) != hipSuccess) return bail("memset", e);
     if ((e = hipMemsetAsync(c->d_jst, 0, sizeof(double) * W * np, c->stream)) != hipSuccess) return bail("memset", e);
@@ -380,6 +475,15 @@ int hh_set_tuning(hh_ctx *c, int kernel, int jsplit)
     if (!c || kernel < 0 || kernel > 2 || jsplit < 0) return HH_E_ARG;
     c->tune_kernel = kernel;
     c->tune_split = jsplit;
+    return HH_OK;
+}
+
+int hh_set_tuning_ex(hh_ctx *c, int i_per_thread, int split_fold)
+{
+    if (!c || i_per_thread < 0 || i_per_thread > 2) return HH_E_ARG;
+    if (split_fold != 0 && split_fold != 4 && split_fold != 8 && split_fold != 16 && split_fold != 32) return HH_E_ARG;
+    c->tune_ni = i_per_thread;
+    c->tune_fold = split_fold;
     return HH_OK;
 }
 

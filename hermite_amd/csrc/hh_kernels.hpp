@@ -149,6 +149,7 @@ struct TileArgs {
     int pot_mask_r2;                 // potential self-exclusion by r2 > eps2 (order 4)
     double *__restrict__ part;
     unsigned *__restrict__ cnt;
+    int last_block;                  // nslice > 1: the last workgroup of an i-block sums the slices
     double *__restrict__ fout;       // [nact][F] (ORDER > 0) or [nact] (ORDER 0)
     double *__restrict__ pout;       // fused potential [nact] or null
 };
@@ -365,8 +366,8 @@ __device__ __forceinline__ void body_split(const double *p, const double *q, flo
     }
 }
 
-// j per FP32 partial sum in the SPLIT body before it is folded into FP64.
-constexpr int SPLIT_FOLD = 32;
+// SPLIT: j per FP32 partial sum before it is folded into the FP64
+// accumulators (a template parameter FOLD of tile_kernel, tuned per order).
 
 // ---- i-parallel tile kernel ----------------------------------------------
 // Workgroup (iblock, slice): NT threads each own NI i-particles
@@ -376,9 +377,10 @@ constexpr int SPLIT_FOLD = 32;
 // With nslice > 1 every workgroup writes its partial sums; the last one to
 // finish an i-block (device-scope counter) adds the slices in slice order,
 // so the result does not depend on scheduling.  ORDER 0 is the potential.
-template <int ORDER, int MODE, int NI, bool POT>
+template <int ORDER, int MODE, int NI, bool POT, int FOLD>
 __global__ __launch_bounds__(NT) void tile_kernel(TileArgs a)
 {
+    static_assert(TJ % FOLD == 0, "FOLD must divide the tile");
     constexpr int Q = pr_words(ORDER);
     constexpr int L = lds_words(ORDER);
     constexpr int NC = n_acc(ORDER, POT);
@@ -447,14 +449,14 @@ __global__ __launch_bounds__(NT) void tile_kernel(TileArgs a)
                     body_fp64<ORDER, POT>(pi[k], q, a.eps2, jbase + jj == ii[k], a.pot_mask_r2, A[k]);
             }
         } else {
-            for (int jb = 0; jb < TJ; jb += SPLIT_FOLD) {
+            for (int jb = 0; jb < TJ; jb += FOLD) {
                 float Af[NI][NC];
 #pragma unroll
                 for (int k = 0; k < NI; k++)
 #pragma unroll
                     for (int c = 0; c < NC; c++) Af[k][c] = 0.0f;
 #pragma unroll 2
-                for (int jj = jb; jj < jb + SPLIT_FOLD; jj++) {
+                for (int jj = jb; jj < jb + FOLD; jj++) {
                     double q[L];
 #pragma unroll
                     for (int w = 0; w < L; w++) q[w] = sj[jj * L + w];
@@ -487,26 +489,189 @@ __global__ __launch_bounds__(NT) void tile_kernel(TileArgs a)
     for (int k = 0; k < NI; k++)
 #pragma unroll
         for (int c = 0; c < NC; c++) a.part[((size_t)slice * NC + c) * ipad + ii[k]] = A[k][c];
+    if (!a.last_block) return;  // slice_reduce_kernel adds the slices
     __threadfence();
     __syncthreads();
     if (tid == 0) s_last = (atomicAdd(&a.cnt[iblock], 1u) == (unsigned)(a.nslice - 1));
     __syncthreads();
     if (!s_last) return;
+    // Acquire: the fence invalidates this CU's L1, so the plain loads below
+    // see every slice's partials (released by the writers' fence + atomic).
     __threadfence();
 #pragma unroll
     for (int k = 0; k < NI; k++) {
         if (ii[k] >= a.nact) continue;
+        double sum[NC];
+#pragma unroll
+        for (int c = 0; c < NC; c++) sum[c] = 0.0;
+        for (int sl = 0; sl < a.nslice; sl++) {
+#pragma unroll
+            for (int c = 0; c < NC; c++) sum[c] += a.part[((size_t)sl * NC + c) * ipad + ii[k]];
+        }
 #pragma unroll
         for (int c = 0; c < NC; c++) {
-            double s = 0.0;
-            for (int sl = 0; sl < a.nslice; sl++)
-                s += __hip_atomic_load(&a.part[((size_t)sl * NC + c) * ipad + ii[k]], __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-            if (c < F) a.fout[(size_t)ii[k] * F + c] = s;
-            else a.pout[ii[k]] = s;
+            if (c < F) a.fout[(size_t)ii[k] * F + c] = sum[c];
+            else a.pout[ii[k]] = sum[c];
         }
     }
     if (tid == 0) a.cnt[iblock] = 0u;
+}
+
+// Sum of the j-slice partials of tile_kernel in slice order (the same order
+// as its last-workgroup path), one thread per (component, i): the reduction
+// of SRC/hermite6-titan.cu:441-459 without its warp-shuffle tree, since
+// every thread here owns a whole sum.
+__global__ __launch_bounds__(256) void slice_reduce_kernel(const double *__restrict__ part, int nslice, int nc,
+                                                           int f, int ipad, int nact, double *__restrict__ fout,
+                                                           double *__restrict__ pout)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nact * nc) return;
+    const int c = t / nact, i = t - c * nact;
+    double s = 0.0;
+    for (int sl = 0; sl < nslice; sl++) s += part[((size_t)sl * nc + c) * ipad + i];
+    if (c < f) fout[(size_t)i * f + c] = s;
+    else pout[i] = s;
+}
+
+// ---- j-parallel kernel for short active lists ----------------------------
+// For nact much smaller than a workgroup the i-parallel kernel leaves most
+// lanes idle and serialises a whole j-tile per wave.  Here lanes take j
+// (coalesced SoA loads straight from HBM, no LDS staging) and every lane
+// holds a group of GI i-particles (uniform across the workgroup, read with
+// scalar-friendly broadcast loads).  Per-lane sums are reduced across the
+// wave with xor-shuffles, across the 4 waves through LDS, and across the
+// j-slices by the last workgroup of each i-group in slice order
+// (deterministic).  FP64 and SPLIT bodies only: the EXACT variant must sum j
+// in index order and uses tile_kernel with one slice.
+struct JParArgs {
+    const double *__restrict__ src;
+    int npad;
+    int nact;
+    int ngroups;     // ceil(nact / GI)
+    int jchunk;      // j per workgroup, a multiple of NT
+    int nslice;
+    double eps2;
+    double *__restrict__ part;   // [(slice*ngroups + group)*V + v]
+    unsigned *__restrict__ cnt;  // per i-group arrival counters
+    double *__restrict__ fout;
+    double *__restrict__ pout;
+};
+
+__device__ __forceinline__ double wave_sum(double x)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
+template <int ORDER, int MODE, int GI, bool POT>
+__global__ __launch_bounds__(NT) void jpar_kernel(JParArgs a)
+{
+    static_assert(MODE != EXACT, "EXACT sums j in index order: use tile_kernel");
+    constexpr int Q = pr_words(ORDER);
+    constexpr int NC = n_acc(ORDER, POT);
+    constexpr int V = GI * NC;
+    constexpr int F = f_words(ORDER);
+    constexpr int NW = NT / 64;
+    static_assert(V <= NT, "too many values per group");
+    __shared__ double red[NW][V];
+    __shared__ double red2[NT];
+    __shared__ unsigned s_last;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int group = blockIdx.x, slice = blockIdx.y;
+    const int npad = a.npad;
+
+    double pi[GI][Q];
+    int ii[GI];
+#pragma unroll
+    for (int g = 0; g < GI; g++) {
+        ii[g] = group * GI + g;
+        const int idx = ii[g] < a.nact ? ii[g] : a.nact - 1;
+#pragma unroll
+        for (int w = 0; w < Q; w++) pi[g][w] = a.src[(size_t)w * npad + idx];
+    }
+    double A[GI][NC];
+#pragma unroll
+    for (int g = 0; g < GI; g++)
+#pragma unroll
+        for (int c = 0; c < NC; c++) A[g][c] = 0.0;
+
+    const int j0 = slice * a.jchunk;
+    const int j1 = min(j0 + a.jchunk, npad);
+    const float eps2f = (float)a.eps2;
+    for (int j = j0 + tid; j < j1; j += NT) {
+        double q[Q];
+#pragma unroll
+        for (int w = 0; w < Q; w++) q[w] = a.src[(size_t)w * npad + j];
+#pragma unroll
+        for (int g = 0; g < GI; g++) {
+            if constexpr (MODE == FP64) {
+                body_fp64<ORDER, POT>(pi[g], q, a.eps2, j == ii[g], 0, A[g]);
+            } else {
+                float Af[NC];
+#pragma unroll
+                for (int c = 0; c < NC; c++) Af[c] = 0.0f;
+                body_split<ORDER, POT>(pi[g], q, eps2f, j == ii[g], 0, Af);
+#pragma unroll
+                for (int c = 0; c < NC; c++) A[g][c] += (double)Af[c];
+            }
+        }
+    }
+
+    // workgroup sum in a fixed order: shuffle tree per wave, then waves 0..3
+#pragma unroll
+    for (int g = 0; g < GI; g++)
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            const double x = wave_sum(A[g][c]);
+            if (lane == 0) red[wave][g * NC + c] = x;
+        }
+    __syncthreads();
+    double tot = 0.0;
+    if (tid < V) {
+#pragma unroll
+        for (int w = 0; w < NW; w++) tot += red[w][tid];
+    }
+    const int gi = tid / NC, ci = tid - gi * NC;
+    if (a.nslice == 1) {
+        if (tid < V) {
+            const int i = group * GI + gi;
+            if (i < a.nact) {
+                if (ci < F) a.fout[(size_t)i * F + ci] = tot;
+                else a.pout[i] = tot;
+            }
+        }
+        return;
+    }
+    if (tid < V) a.part[((size_t)slice * a.ngroups + group) * V + tid] = tot;
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) s_last = (atomicAdd(&a.cnt[group], 1u) == (unsigned)(a.nslice - 1));
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();  // acquire: see the other slices' partials
+    // last workgroup: sum the slices, NSUB interleaved partial sums per value
+    constexpr int NSUB = NT / V;
+    double s = 0.0;
+    if (tid < V * NSUB) {
+        const int v = tid / NSUB, sub = tid - v * NSUB;
+        for (int sl = sub; sl < a.nslice; sl += NSUB) s += a.part[((size_t)sl * a.ngroups + group) * V + v];
+    }
+    red2[tid] = s;
+    __syncthreads();
+    if (tid < V) {
+        double t = 0.0;
+#pragma unroll
+        for (int sub = 0; sub < NSUB; sub++) t += red2[tid * NSUB + sub];
+        const int i = group * GI + gi;
+        if (i < a.nact) {
+            if (ci < F) a.fout[(size_t)i * F + ci] = t;
+            else a.pout[i] = t;
+        }
+    }
+    if (tid == 0) a.cnt[group] = 0u;
 }
 
 }  // namespace hh

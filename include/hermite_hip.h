@@ -116,6 +116,12 @@ const char *hh_last_error(const hh_ctx *ctx);
  * jsplit: j-slices per i-block (0 = automatic). */
 int hh_set_tuning(hh_ctx *ctx, int kernel, int jsplit);
 
+/* Register blocking and split accumulation: i_per_thread 1 or 2 i-particles
+ * per lane in the i-parallel kernel (0 = default 1); split_fold j per FP32
+ * partial sum of the SPLIT body before it is added to FP64 (4, 8, 16, 32;
+ * 0 = default). */
+int hh_set_tuning_ex(hh_ctx *ctx, int i_per_thread, int split_fold);
+
 /* Runs the force on [0, nact) reps times on the device without copying the
  * result back, and reports the mean device time per call in milliseconds,
  * measured with HIP events on the context's stream. */

@@ -154,6 +154,16 @@ static std::vector<double> read_recs(const char *path, int n)
     return v;
 }
 
+// The AVX back end stores Force records and potentials with aligned 32-byte
+// vector stores (SRC/hermite6-avx.h:95-104,543), as the reference driver's
+// 64-byte aligned buffers allow (SRC/nbodysystem.h:301-304).
+template <typename T>
+struct Aligned {
+    T *p;
+    explicit Aligned(size_t n) : p(allocate<T, 64>(n)) { std::memset((void *)p, 0, n * sizeof(T)); }
+    ~Aligned() { deallocate<T, 64>(p); }
+};
+
 static void write_out(const char *path, const void *p, size_t bytes)
 {
     if (std::string(path) == "-") return;
@@ -175,15 +185,15 @@ int main(int argc, char **argv)
         std::vector<double> recs = read_recs(argv[2], n);
         void *g = ref_gravity_new(n);
         for (int i = 0; i < n; i++) ref_set_jp(g, i, recs.data() + (size_t)i * (sizeof(Particle) / 8));
-        std::vector<double> f((size_t)(nact + 4) * (sizeof(Force) / 8));
+        Aligned<double> f((size_t)(nact + 4) * (sizeof(Force) / 8));
         ref_predict_all(g, tsys);
-        ref_force(g, nact, eps2, f.data());  // warm-up
+        ref_force(g, nact, eps2, f.p);  // warm-up
         const double t0 = omp_get_wtime();
-        for (int r = 0; r < reps; r++) ref_force(g, nact, eps2, f.data());
+        for (int r = 0; r < reps; r++) ref_force(g, nact, eps2, f.p);
         const double t1 = omp_get_wtime();
         std::printf("{\"seconds\": %.9e, \"reps\": %d, \"nact\": %d, \"n\": %d, \"threads\": %d}\n",
                     t1 - t0, reps, nact, n, omp_get_max_threads());
-        write_out(argv[8], f.data(), (size_t)nact * sizeof(Force));
+        write_out(argv[8], f.p, (size_t)nact * sizeof(Force));
         ref_gravity_free(g);
         return 0;
     }
@@ -193,9 +203,9 @@ int main(int argc, char **argv)
         std::vector<double> recs = read_recs(argv[2], n);
         void *g = ref_gravity_new(n);
         for (int i = 0; i < n; i++) ref_set_jp(g, i, recs.data() + (size_t)i * (sizeof(Particle) / 8));
-        std::vector<double> pot(n);
-        ref_potential(g, eps2, pot.data());
-        write_out(argv[5], pot.data(), sizeof(double) * n);
+        Aligned<double> pot((size_t)n + 4);
+        ref_potential(g, eps2, pot.p);
+        write_out(argv[5], pot.p, sizeof(double) * n);
         ref_gravity_free(g);
         return 0;
     }

@@ -1,0 +1,84 @@
+"""The block-step driver on the HIP back end, end to end.
+
+EXACT variant: the whole integration is bit-identical to the oracle's (and so
+to the reference CPU build's) and reproduces the nact known answers printed
+in the reference's run logs.  FP64 / SPLIT: last-bit force differences move
+individual timestep levels, so the comparison is statistical: mean nact
+within 1 % of the known answer and |dE/E| of the same size as the oracle's
+(the split kernel within 2x of the FP64 kernel)."""
+import numpy as np
+import pytest
+
+import hermite_amd as H
+import oracle as O
+from tests.test_golden import NACT_KAT
+
+pytestmark = pytest.mark.gpu
+
+ORDERS = (4, 6, 8)
+
+
+def run_pl1k(order, variant, tend=2.0):
+    eta, eta_s, _, _ = NACT_KAT[order]
+    n = 1024
+    m, x, v = H.plummer(n, 19640916, 100.0)
+    eps = 4.0 / n
+    s = H.System(order, O.make_particles(order, m, x, v), 0.0, eps * eps, eta, eta_s, 1 / 16, variant=variant)
+    s.init_step()
+    s.integrate_until(1.0)
+    ns1, nb1 = s.num_step, s.num_bstep
+    s.integrate_until(tend)
+    return s, ns1 / nb1, (s.num_step - ns1) / (s.num_bstep - nb1)
+
+
+@pytest.mark.parametrize("order", ORDERS)
+def test_exact_driver_reproduces_reference_nact(order):
+    print(f"exact driver order={order}", flush=True)
+    s, a1, a2 = run_pl1k(order, "exact")
+    _, _, want1, want2 = NACT_KAT[order]
+    assert "%e" % a1 == want1 and "%e" % a2 == want2
+    s.close()
+
+
+@pytest.mark.parametrize("order", ORDERS)
+def test_exact_driver_bitwise_vs_oracle(order):
+    n = 256
+    m, x, v = H.plummer(n, 31)
+    P = O.make_particles(order, m, x, v)
+    args = (0.0, (1 / 32) ** 2, {4: 0.1, 6: 0.4, 8: 0.6}[order], {4: 0.01, 6: 0.1, 8: 0.1}[order], 1 / 16)
+    a = O.System(order, P, *args)
+    b = H.System(order, P, *args, variant="exact")
+    a.init_step()
+    b.init_step()
+    assert a.init_energy == b.init_energy
+    for _ in range(4):
+        a.integrate_one_dtmax()
+        b.integrate_one_dtmax()
+        assert (a.num_step, a.num_bstep) == (b.num_step, b.num_bstep)
+        assert np.array_equal(a.ptcl(), b.ptcl())
+    assert a.energy() == b.energy()
+    b.close()
+
+
+@pytest.mark.parametrize("order", ORDERS)
+def test_fp64_and_split_driver_statistics(order):
+    _, _, want1, _ = NACT_KAT[order]
+    res = {}
+    for variant in ("fp64", "split"):
+        s, a1, _ = run_pl1k(order, variant, tend=1.0)
+        de = abs((s.energy() - s.init_energy) / s.init_energy)
+        res[variant] = (a1, de)
+        print(f"order={order} {variant}: nact[0,1]={a1:.6e} (reference {want1}) |dE/E|={de:.3e}", flush=True)
+        s.close()
+        assert abs(a1 / float(want1) - 1) < 0.01
+    # the oracle's own energy error on the same run
+    eta, eta_s, _, _ = NACT_KAT[order]
+    m, x, v = H.plummer(1024, 19640916, 100.0)
+    o = O.System(order, O.make_particles(order, m, x, v), 0.0, (4.0 / 1024) ** 2, eta, eta_s, 1 / 16)
+    o.init_step()
+    while o.tsys < 1.0:
+        o.integrate_one_dtmax()
+    de_o = abs((o.energy() - o.init_energy) / o.init_energy)
+    print(f"order={order} oracle |dE/E|={de_o:.3e}", flush=True)
+    assert res["fp64"][1] < 3 * de_o + 1e-13
+    assert res["split"][1] < 2 * res["fp64"][1] + 1e-12

@@ -1,0 +1,66 @@
+"""Drop-in check: the reference's own NbodySystem driver and Particle physics
+(compiled from the reference sources into oracle/_ref/ref_hipN) running on
+the HIP back end through include/hermite_hip_gravity.h.
+
+With HERMITE_HIP_VARIANT=exact the run reproduces the nact known answers of
+the reference logs and the oracle's energies bit for bit; with fp64 the mean
+nact stays within 1 % of them."""
+import json
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import hermite_amd as H
+import oracle as O
+from tests.test_golden import NACT_KAT
+
+pytestmark = pytest.mark.gpu
+
+ORDERS = (4, 6, 8)
+
+
+def run_dropin(order, variant, P, eps2, eta, eta_s, dtmax, tend):
+    exe = os.path.join(O.REF_DIR, f"ref_hip{order}")
+    if not os.path.exists(exe):
+        pytest.skip("oracle/_ref/ref_hipN not built (needs the reference sources)")
+    env = dict(os.environ, HERMITE_HIP_VARIANT=variant)
+    with tempfile.TemporaryDirectory() as td:
+        fin = os.path.join(td, "p.bin")
+        np.ascontiguousarray(P).tofile(fin)
+        out = subprocess.run([exe, "run", fin, str(len(P)), repr(eps2), repr(eta), repr(eta_s), repr(dtmax),
+                              repr(tend)], capture_output=True, text=True, env=env, timeout=600)
+    assert out.returncode == 0, out.stderr[-2000:]
+    return json.loads([ln for ln in out.stdout.splitlines() if ln.startswith("{")][-1])
+
+
+@pytest.mark.parametrize("order", ORDERS)
+def test_reference_driver_on_hip_exact(order):
+    eta, eta_s, want1, want2 = NACT_KAT[order]
+    n = 1024
+    m, x, v = H.plummer(n, 19640916, 100.0)
+    P = O.make_particles(order, m, x, v)
+    eps2 = (4.0 / n) ** 2
+    print(f"drop-in exact order={order}", flush=True)
+    r = run_dropin(order, "exact", P, eps2, eta, eta_s, 1 / 16, 2.0)
+    assert "%e" % (r["num_step_t1"] / r["num_bstep_t1"]) == want1
+    assert "%e" % ((r["num_step"] - r["num_step_t1"]) / (r["num_bstep"] - r["num_bstep_t1"])) == want2
+    o = O.System(order, P, 0.0, eps2, eta, eta_s, 1 / 16)
+    o.init_step()
+    while o.tsys < 2.0:
+        o.integrate_one_dtmax()
+    assert r["e0"] == o.init_energy and r["e1"] == o.energy()
+
+
+@pytest.mark.parametrize("order", ORDERS)
+def test_reference_driver_on_hip_fp64(order):
+    eta, eta_s, want1, _ = NACT_KAT[order]
+    n = 1024
+    m, x, v = H.plummer(n, 19640916, 100.0)
+    P = O.make_particles(order, m, x, v)
+    r = run_dropin(order, "fp64", P, (4.0 / n) ** 2, eta, eta_s, 1 / 16, 1.0)
+    nact = r["num_step"] / r["num_bstep"]
+    print(f"drop-in fp64 order={order} nact={nact:.6e} (reference {want1}) dE/E={(r['e1'] - r['e0']) / r['e0']:.3e}")
+    assert abs(nact / float(want1) - 1) < 0.01
