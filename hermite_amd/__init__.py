@@ -63,6 +63,19 @@ def hip_lib():
         L.hh_predict_async.argtypes = [vp, C.c_double]
         L.hh_synchronize.argtypes = [vp]
         L.hh_last_geometry.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.hhr_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), _dp, _dp, C.POINTER(vp)]
+        L.hhr_destroy.argtypes = [vp]
+        L.hhr_run.argtypes = [vp, C.c_double, C.c_int, C.c_int]
+        L.hhr_stats.argtypes = [vp, C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+                                C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
+        L.hhr_last_error.argtypes = [vp]
+        L.hhr_last_error.restype = C.c_char_p
+        L.hhr_last_run_ms.argtypes = [vp]
+        L.hhr_last_run_ms.restype = C.c_double
+        L.hhr_steps_launched.argtypes = [vp]
+        L.hhr_steps_launched.restype = C.c_longlong
+        L.hhr_get_particles.argtypes = [vp, C.c_int, _dp]
+        L.hhr_energy.argtypes = [vp, C.c_int, _dp]
         _hip = L
     return _hip
 
@@ -302,3 +315,62 @@ class System:
         out = np.zeros(len(self.PHASES))
         host_lib().hhd_profile(self.h, _p(out))
         return dict(zip(self.PHASES, out.tolist()))
+
+
+# ------------------------------------------------------------------ resident
+class Resident:
+    """GPU-resident block-step integration of one or many independent clusters
+    (hhr_* in libhermite_hip.so): schedule, prediction, force and corrector
+    all on the device, optionally replayed from a hipGraph.
+
+    clusters: list of (P, eps2, eta, dtmax, dtmin, tsys) where P are the
+    cluster's Particle records after init_step (e.g. System(...).ptcl())."""
+
+    def __init__(self, order, variant, clusters, device=0):
+        L = hip_lib()
+        self.order, self.variant = order, variant
+        self.ns = [len(c[0]) for c in clusters]
+        recs = np.ascontiguousarray(np.concatenate([np.asarray(c[0], dtype=np.float64) for c in clusters]))
+        params = np.ascontiguousarray(np.array([[c[1], c[2], c[3], c[4], c[5]] for c in clusters], dtype=np.float64))
+        ns = (C.c_int * len(self.ns))(*self.ns)
+        h = C.c_void_p()
+        rc = L.hhr_create(order, VARIANTS[variant], device, len(self.ns), ns, _p(recs), _p(params), C.byref(h))
+        if rc != 0:
+            raise HHError(f"hhr_create failed ({rc}): {L.hhr_last_error(None).decode()}")
+        self.h = h
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise HHError(f"{what} failed ({rc}): {hip_lib().hhr_last_error(self.h).decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            hip_lib().hhr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def run(self, tend, use_graph=True, batch=8):
+        self._check(hip_lib().hhr_run(self.h, tend, int(use_graph), batch), "hhr_run")
+        return hip_lib().hhr_last_run_ms(self.h)
+
+    def stats(self, k=0):
+        a, b, d = C.c_longlong(), C.c_longlong(), C.c_longlong()
+        t = C.c_double()
+        self._check(hip_lib().hhr_stats(self.h, k, C.byref(a), C.byref(b), C.byref(t), C.byref(d)), "hhr_stats")
+        return {"num_step": a.value, "num_bstep": b.value, "tsys": t.value, "dtmin_hits": d.value}
+
+    def energy(self, k=0):
+        e = C.c_double()
+        self._check(hip_lib().hhr_energy(self.h, k, C.byref(e)), "hhr_energy")
+        return e.value
+
+    def particles(self, k=0):
+        out = np.zeros((self.ns[k], PSTRIDE[self.order]))
+        self._check(hip_lib().hhr_get_particles(self.h, k, _p(out)), "hhr_get_particles")
+        return out
+
+    @property
+    def steps_launched(self):
+        return hip_lib().hhr_steps_launched(self.h)

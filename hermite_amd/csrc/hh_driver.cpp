@@ -344,14 +344,17 @@ struct hhd_sys {
         for (int i = 0; i < nact; i++) dtbuf[i] = P(i)[DT] + P(i)[TLAST];
     }
 
+    // Largest power of two <= dtmax dividing tnext (SRC/nbodysystem.h:652-661).
+    // Returns 0 when tnext is not on the power-of-two grid (a collapsed step).
     double calc_dtlim(double tnext) const
     {
         double dtlim = dtmax, s = tnext / dtmax;
-        while (s != std::floor(s)) {
+        for (int k = 0; k < 1100 && std::isfinite(s); k++) {
+            if (s == std::floor(s)) return dtlim;
             s *= 2.0;
             dtlim *= 0.5;
         }
-        return dtlim;
+        return 0.0;
     }
 
     int count_nact(double tnext) const
@@ -415,22 +418,41 @@ struct hhd_sys {
         const double tnext = P(0)[TLAST] + P(0)[DT];
         const double dtlim = calc_dtlim(tnext);
         const int nact = count_nact(tnext);
+        // A non-finite or non-advancing block time means some particle's step
+        // collapsed (the quantised Aarseth step of noisy derivatives can run
+        // away); stop with an error instead of looping.
+        if (!(tnext > tsys) || !std::isfinite(tnext) || dtlim <= 0.0 || nact <= 0) {
+            char msg[160];
+            std::snprintf(msg, sizeof msg, "block time does not advance: tsys=%.17g tnext=%.17g nact=%d", tsys,
+                          tnext, nact);
+            if (rc == HH_OK) {
+                rc = HH_E_STATE;
+                err = msg;
+            }
+            return 0;
+        }
         double t1 = omp_get_wtime();
         ok(hh_predict_all(g, tnext), "hh_predict_all");
         double t2 = omp_get_wtime();
         ok(hh_force_first_nact(g, nact, eps2, force.data(), nullptr), "hh_force_first_nact");
         double t3 = omp_get_wtime();
-        long hits = 0;
-#pragma omp parallel for reduction(+ : hits) if (nact > 256)
+        long hits = 0, bad = 0;
+#pragma omp parallel for reduction(+ : hits, bad) if (nact > 256)
         for (int i = 0; i < nact; i++) {
             correct(i, dtlim);
             if (dtmin > 0.0 && P(i)[DT] < dtmin) {
                 P(i)[DT] = dtmin;
                 hits++;
             }
+            if (!std::isfinite(P(i)[POS] + P(i)[POS + 1] + P(i)[POS + 2] + P(i)[VEL] + P(i)[VEL + 1] + P(i)[VEL + 2]))
+                bad++;
             dtbuf[i] = P(i)[DT];
         }
         dtmin_hits += hits;
+        if (bad && rc == HH_OK) {
+            rc = HH_E_STATE;
+            err = "a particle state became non-finite";
+        }
         double t4 = omp_get_wtime();
         sort_by_level_cached(nact, dtlim);
         double t5 = omp_get_wtime();

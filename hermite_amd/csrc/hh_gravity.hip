@@ -11,6 +11,7 @@
 #include "../../include/hermite_hip.h"
 
 #include <algorithm>
+#include <limits>
 #include <atomic>
 #include <cstdio>
 #include <cstring>
@@ -32,8 +33,6 @@ struct hh_ctx {
     double *d_pot = nullptr;    // npad
     double *d_part = nullptr;   // partial sums of j-sliced launches
     size_t part_cap = 0;        // doubles
-    unsigned *d_cnt = nullptr;  // per-i-block arrival counters
-    int cnt_cap = 0;
     std::atomic<int> dirty_hi{0};
     int tune_kernel = 0, tune_split = 0, tune_ni = 0, tune_fold = 0;
     int last_kernel = 0, last_split = 0, last_wgs = 0;
@@ -79,7 +78,6 @@ static void free_all(hh_ctx *c)
     (void)hipFree(c->d_force);
     (void)hipFree(c->d_pot);
     (void)hipFree(c->d_part);
-    (void)hipFree(c->d_cnt);
     (void)hipHostFree(c->h_jp);
     (void)hipHostFree(c->h_force);
     (void)hipHostFree(c->h_pot);
@@ -149,9 +147,16 @@ static int ensure_part(hh_ctx *c, size_t need)
 
 // Workgroups wanted per force launch: several per CU on 256 CUs.
 static constexpr int TARGET_WG = 2048;
-// Up to this many j-slices the last workgroup of each i-block adds them;
-// beyond it a separate slice_reduce_kernel does, with the whole GPU.
-static constexpr int LAST_BLOCK_MAX = 16;
+
+static hipError_t launch_reduce(hh_ctx *c, const double *part, int nslice, int nc, int f, int ipad, int ni,
+                                double *fout, double *pout)
+{
+    const int per_block = 256 / RL;
+    const int nt = ni * nc;
+    hipLaunchKernelGGL(slice_reduce_kernel, dim3((nt + per_block - 1) / per_block), dim3(256), 0, c->stream, part,
+                       nslice, nc, f, ipad, ni, fout, pout);
+    return hipGetLastError();
+}
 
 // i-parallel launch of tile_kernel over src slots [0, ni) as i-particles.
 template <int ORDER, int MODE, bool POT, int NI, int FOLD>
@@ -180,24 +185,16 @@ static int launch_tile_ni(hh_ctx *c, const double *src, int ni, double eps2, dou
     a.eps2 = eps2;
     a.pot_mask_r2 = (ORDER == 0 && c->order == 4) ? 1 : 0;
     a.part = nullptr;
-    a.cnt = c->d_cnt;
-    a.last_block = nslice <= LAST_BLOCK_MAX;
     a.fout = fout;
     a.pout = pout;
     if (nslice > 1) {
-        if (iblocks > c->cnt_cap) return fail(c, HH_E_STATE, "counter buffer too small");
         const int r = ensure_part(c, (size_t)nslice * NC * a.ipad);
         if (r) return r;
         a.part = c->d_part;
     }
     hipLaunchKernelGGL((tile_kernel<ORDER, MODE, NI, POT, FOLD>), dim3(iblocks, nslice), dim3(NT), 0, c->stream, a);
     hipError_t e = hipGetLastError();
-    if (e == hipSuccess && nslice > 1 && !a.last_block) {
-        const int nt = ni * NC;
-        hipLaunchKernelGGL(slice_reduce_kernel, dim3((nt + 255) / 256), dim3(256), 0, c->stream,
-                           (const double *)a.part, nslice, NC, ORDER == 0 ? 1 : f_words(ORDER), a.ipad, ni, fout, pout);
-        e = hipGetLastError();
-    }
+    if (e == hipSuccess && nslice > 1) e = launch_reduce(c, a.part, nslice, NC, ORDER == 0 ? 1 : f_words(ORDER), a.ipad, ni, fout, pout);
     if (e != hipSuccess) return fail(c, HH_E_HIP, std::string("force launch: ") + hipGetErrorString(e));
     c->last_kernel = 1;
     c->last_split = nslice;
@@ -259,17 +256,18 @@ static int launch_jpar(hh_ctx *c, int nact, double eps2)
     a.nslice = nslice;
     a.eps2 = eps2;
     a.part = nullptr;
-    a.cnt = c->d_cnt;
     a.fout = c->d_force;
     a.pout = c->d_pot;
     if (nslice > 1) {
-        if (ngroups > c->cnt_cap) return fail(c, HH_E_STATE, "counter buffer too small");
-        const int r = ensure_part(c, (size_t)nslice * ngroups * V);
+        const int r = ensure_part(c, (size_t)nslice * V * ngroups);
         if (r) return r;
         a.part = c->d_part;
     }
     hipLaunchKernelGGL((jpar_kernel<ORDER, MODE, GI, POT>), dim3(ngroups, nslice), dim3(NT), 0, c->stream, a);
-    const hipError_t e = hipGetLastError();
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && nslice > 1)
+        e = launch_reduce(c, a.part, nslice, n_acc(ORDER, POT), f_words(ORDER), ngroups * GI, nact, c->d_force,
+                          c->d_pot);
     if (e != hipSuccess) return fail(c, HH_E_HIP, std::string("force launch: ") + hipGetErrorString(e));
     c->last_kernel = 2;
     c->last_split = nslice;
@@ -380,9 +378,6 @@ int hh_create(int order, int nbody, int variant, int device, hh_ctx **out)
     if ((e = hipMalloc(&c->d_jpr, sizeof(double) * Q * np)) != hipSuccess) return bail("predictions", e);
     if ((e = hipMalloc(&c->d_force, sizeof(double) * F * nbody)) != hipSuccess) return bail("force", e);
     if ((e = hipMalloc(&c->d_pot, sizeof(double) * np)) != hipSuccess) return bail("pot", e);
-    c->cnt_cap = (int)np;  // one per i-group of the finest grouping
-    if ((e = hipMalloc(&c->d_cnt, sizeof(unsigned) * c->cnt_cap)) != hipSuccess) return bail("counters", e);
-    if ((e = hipMemsetAsync(c->d_cnt, 0, sizeof(unsigned) * c->cnt_cap, c->stream)) != hipSuccess) return bail("memset", e);
     if ((e = hipMemsetAsync(c->d_jst, 0, sizeof(double) * W * np, c->stream)) != hipSuccess) return bail("memset", e);
     std::memset(c->h_jp, 0, sizeof(double) * W * nbody);
     switch (order) {
@@ -551,6 +546,417 @@ int hh_last_geometry(const hh_ctx *c, int *kernel, int *jsplit, int *workgroups)
     if (kernel) *kernel = c->last_kernel;
     if (jsplit) *jsplit = c->last_split;
     if (workgroups) *workgroups = c->last_wgs;
+    return HH_OK;
+}
+
+}  // extern "C"
+
+// =====================================================================
+// GPU-resident block-step driver for one or many clusters (hhr_*).
+// =====================================================================
+#include "hh_resident.hpp"
+
+#include <cmath>
+#include <vector>
+
+struct hhr_ctx {
+    int order = 0, variant = 0, device = 0, nclu = 0, npad = 0, ntiles = 0, rw = 0;
+    int force_grid = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double *rp = nullptr, *jpr = nullptr, *tn = nullptr, *part = nullptr, *pot = nullptr;
+    int *ilist = nullptr, *tile_clu = nullptr;
+    RClu *clu = nullptr;
+    RGlob *glob = nullptr;
+    size_t part_cap = 0;
+    std::vector<int> off, n;
+    std::vector<double> eps2;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int graph_steps = 0;
+    double last_run_ms = 0.0;
+    long long steps_launched = 0;
+    std::string err;
+};
+
+static std::string g_hhr_err;
+
+static int rfail(hhr_ctx *c, int code, const std::string &m)
+{
+    if (c) c->err = m;
+    else g_hhr_err = m;
+    return code;
+}
+
+#define RCHK(c, call)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return rfail((c), HH_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+static RArgs r_args(hhr_ctx *c)
+{
+    RArgs a;
+    a.rp = c->rp;
+    a.jpr = c->jpr;
+    a.tn = c->tn;
+    a.ilist = c->ilist;
+    a.tile_clu = c->tile_clu;
+    a.clu = c->clu;
+    a.glob = c->glob;
+    a.part = c->part;
+    a.npad = c->npad;
+    a.ntiles_total = c->ntiles;
+    return a;
+}
+
+template <int ORDER, int MODE>
+static hipError_t r_launch_step(hhr_ctx *c)
+{
+    const RArgs a = r_args(c);
+    hipLaunchKernelGGL(r_min_kernel, dim3(c->ntiles), dim3(NT), 0, c->stream, a);
+    hipLaunchKernelGGL(r_compact_kernel, dim3(c->ntiles), dim3(NT), 0, c->stream, a);
+    hipLaunchKernelGGL((r_plan_kernel<ORDER>), dim3(1), dim3(NT), 0, c->stream, a);
+    hipLaunchKernelGGL((r_predict_kernel<ORDER, MODE == EXACT>), dim3(c->ntiles), dim3(NT), 0, c->stream, a);
+    hipLaunchKernelGGL((r_force_kernel<ORDER, MODE>), dim3(c->force_grid), dim3(NT), 0, c->stream, a);
+    hipLaunchKernelGGL((r_reduce_correct_kernel<ORDER>), dim3(c->force_grid), dim3(NT), 0, c->stream, a);
+    return hipGetLastError();
+}
+
+static hipError_t r_step(hhr_ctx *c)
+{
+    switch (c->order * 4 + c->variant) {
+    case 16 + HH_FP64: return r_launch_step<4, FP64>(c);
+    case 16 + HH_SPLIT: return r_launch_step<4, SPLIT>(c);
+    case 16 + HH_EXACT: return r_launch_step<4, EXACT>(c);
+    case 24 + HH_FP64: return r_launch_step<6, FP64>(c);
+    case 24 + HH_SPLIT: return r_launch_step<6, SPLIT>(c);
+    case 24 + HH_EXACT: return r_launch_step<6, EXACT>(c);
+    case 32 + HH_FP64: return r_launch_step<8, FP64>(c);
+    case 32 + HH_SPLIT: return r_launch_step<8, SPLIT>(c);
+    default: return r_launch_step<8, EXACT>(c);
+    }
+}
+
+static void r_free(hhr_ctx *c)
+{
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    (void)hipFree(c->rp);
+    (void)hipFree(c->jpr);
+    (void)hipFree(c->tn);
+    (void)hipFree(c->part);
+    (void)hipFree(c->pot);
+    (void)hipFree(c->ilist);
+    (void)hipFree(c->tile_clu);
+    (void)hipFree(c->clu);
+    (void)hipFree(c->glob);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+extern "C" {
+
+const char *hhr_last_error(const hhr_ctx *c) { return c ? c->err.c_str() : g_hhr_err.c_str(); }
+
+// recs: the clusters' reference Particle records, concatenated, after
+// init_step (stored derivatives and dt set).  params: per cluster
+// {eps2, eta, dtmax, dtmin, tsys}.
+int hhr_create(int order, int variant, int device, int nclu, const int *ns, const double *recs,
+               const double *params, hhr_ctx **out)
+{
+    if (!out || !ns || !recs || !params || nclu <= 0) return rfail(nullptr, HH_E_ARG, "null argument");
+    *out = nullptr;
+    if (order != 4 && order != 6 && order != 8) return rfail(nullptr, HH_E_ARG, "order must be 4, 6 or 8");
+    if (variant < HH_FP64 || variant > HH_EXACT) return rfail(nullptr, HH_E_ARG, "unknown variant");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return rfail(nullptr, HH_E_NODEV, "no HIP device");
+    if (device < 0 || device >= ndev) return rfail(nullptr, HH_E_ARG, "device ordinal out of range");
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess || std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return rfail(nullptr, HH_E_NODEV, "device is not gfx950");
+    if (hipSetDevice(device) != hipSuccess) return rfail(nullptr, HH_E_NODEV, "hipSetDevice failed");
+
+    hhr_ctx *c = new hhr_ctx();
+    c->order = order;
+    c->variant = variant;
+    c->device = device;
+    c->nclu = nclu;
+    c->rw = r_words(order);
+    c->off.resize(nclu);
+    c->n.assign(ns, ns + nclu);
+    c->eps2.resize(nclu);
+    int tot = 0;
+    for (int k = 0; k < nclu; k++) {
+        if (ns[k] <= 0) {
+            delete c;
+            return rfail(nullptr, HH_E_ARG, "cluster sizes must be positive");
+        }
+        c->off[k] = tot;
+        tot += (ns[k] + TJ - 1) / TJ * TJ;
+    }
+    c->npad = tot;
+    c->ntiles = tot / TJ;
+    const int pw = p_words(order), rw = c->rw, Q = pr_words(order), F = f_words(order);
+
+    // host SoA image of the resident records
+    std::vector<double> img((size_t)rw * tot, 0.0), tn(tot, std::numeric_limits<double>::infinity());
+    std::vector<int> tclu(c->ntiles);
+    std::vector<RClu> hc(nclu);
+    size_t src = 0;
+    for (int k = 0; k < nclu; k++) {
+        const int o = c->off[k], nk = ns[k];
+        const int ntk = (nk + TJ - 1) / TJ;
+        for (int t = 0; t < ntk; t++) tclu[o / TJ + t] = k;
+        for (int i = 0; i < (ntk * TJ); i++) {
+            const size_t s = (size_t)o + i;
+            if (i >= nk) {  // padding: massless, far away, never active
+                for (int w = 0; w < 3; w++) img[(size_t)w * tot + s] = FAR;
+                continue;
+            }
+            const double *p = recs + (src + i) * pw;
+            auto put = [&](int w, double v) { img[(size_t)w * tot + s] = v; };
+            for (int d = 0; d < 3; d++) {
+                put(0 + d, p[4 + d]);    // pos
+                put(4 + d, p[7 + d]);    // vel
+                put(8 + d, p[10 + d]);   // acc
+                put(11 + d, p[13 + d]);  // jrk
+                put(14 + d, p[16 + d]);  // snp
+                put(17 + d, p[19 + d]);  // crk
+                if (order >= 6) {
+                    put(20 + d, p[22 + d]);  // d4a
+                    put(23 + d, p[25 + d]);  // d5a
+                }
+                if (order == 8) {
+                    put(R_D6A + d, p[28 + d]);
+                    put(R_D7A + d, p[31 + d]);
+                }
+            }
+            put(3, p[1]);     // mass
+            put(7, p[2]);     // tlast
+            put(R_DT, p[3]);  // dt
+            put(R_ID, p[0]);  // id bits
+            tn[s] = p[2] + p[3];
+        }
+        src += nk;
+        const double *pr = params + 5 * k;
+        RClu &h = hc[k];
+        std::memset(&h, 0, sizeof(h));
+        h.eps2 = pr[0];
+        h.etapow = std::pow(pr[1], double(2 * (order - 3)));
+        h.dtmax = pr[2];
+        h.dtmin = pr[3];
+        h.tsys = pr[4];
+        h.tend = pr[4];
+        h.tnext = pr[4];
+        h.tmin_bits = 0x7ff0000000000000ULL;
+        h.off = o;
+        h.n = nk;
+        h.ntiles = ntk;
+        c->eps2[k] = pr[0];
+    }
+    RGlob g;
+    std::memset(&g, 0, sizeof(g));
+    g.nclu = nclu;
+    g.exact = variant == HH_EXACT;
+
+    auto bail = [&](const char *what, hipError_t e) {
+        g_hhr_err = std::string(what) + ": " + hipGetErrorString(e);
+        r_free(c);
+        delete c;
+        return HH_E_HIP;
+    };
+    hipError_t e;
+    // partials: one slice of every active particle (nslice 1) or, when
+    // sliced, fewer than two budgets of tasks per cluster (plan kernel)
+    c->part_cap = (size_t)F * (tot + (size_t)NT * nclu) + (size_t)F * NT * 2 * (R_TASK_TARGET + nclu);
+    if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) return bail("stream", e);
+    if ((e = hipEventCreate(&c->ev0)) != hipSuccess) return bail("event", e);
+    if ((e = hipEventCreate(&c->ev1)) != hipSuccess) return bail("event", e);
+    if ((e = hipMalloc(&c->rp, sizeof(double) * img.size())) != hipSuccess) return bail("records", e);
+    if ((e = hipMalloc(&c->jpr, sizeof(double) * (size_t)Q * tot)) != hipSuccess) return bail("predictions", e);
+    if ((e = hipMalloc(&c->tn, sizeof(double) * tot)) != hipSuccess) return bail("tn", e);
+    if ((e = hipMalloc(&c->pot, sizeof(double) * tot)) != hipSuccess) return bail("pot", e);
+    if ((e = hipMalloc(&c->part, sizeof(double) * c->part_cap)) != hipSuccess) return bail("partials", e);
+    if ((e = hipMalloc(&c->ilist, sizeof(int) * tot)) != hipSuccess) return bail("ilist", e);
+    if ((e = hipMalloc(&c->tile_clu, sizeof(int) * c->ntiles)) != hipSuccess) return bail("tile map", e);
+    if ((e = hipMalloc(&c->clu, sizeof(RClu) * nclu)) != hipSuccess) return bail("clusters", e);
+    if ((e = hipMalloc(&c->glob, sizeof(RGlob))) != hipSuccess) return bail("glob", e);
+    if ((e = hipMemcpy(c->rp, img.data(), sizeof(double) * img.size(), hipMemcpyHostToDevice)) != hipSuccess) return bail("upload", e);
+    if ((e = hipMemcpy(c->tn, tn.data(), sizeof(double) * tot, hipMemcpyHostToDevice)) != hipSuccess) return bail("upload", e);
+    if ((e = hipMemcpy(c->tile_clu, tclu.data(), sizeof(int) * c->ntiles, hipMemcpyHostToDevice)) != hipSuccess) return bail("upload", e);
+    if ((e = hipMemcpy(c->clu, hc.data(), sizeof(RClu) * nclu, hipMemcpyHostToDevice)) != hipSuccess) return bail("upload", e);
+    if ((e = hipMemcpy(c->glob, &g, sizeof(RGlob), hipMemcpyHostToDevice)) != hipSuccess) return bail("upload", e);
+    if ((e = hipMemset(c->jpr, 0, sizeof(double) * (size_t)Q * tot)) != hipSuccess) return bail("memset", e);
+    c->force_grid = prop.multiProcessorCount * 4;
+    *out = c;
+    return HH_OK;
+}
+
+int hhr_destroy(hhr_ctx *c)
+{
+    if (!c) return HH_E_ARG;
+    (void)hipSetDevice(c->device);
+    r_free(c);
+    delete c;
+    return HH_OK;
+}
+
+// Integrate every cluster to tend (a multiple of its dt_max).  Steps are
+// launched in batches of batch_steps (captured once into a hipGraph when
+// use_graph != 0); after each batch the number of clusters still active is
+// read back.  Extra steps after the last cluster finishes are no-ops.
+int hhr_run(hhr_ctx *c, double tend, int use_graph, int batch_steps)
+{
+    if (!c || batch_steps <= 0) return HH_E_ARG;
+    RCHK(c, hipSetDevice(c->device));
+    // set tend for every cluster
+    std::vector<RClu> hc(c->nclu);
+    RCHK(c, hipMemcpy(hc.data(), c->clu, sizeof(RClu) * c->nclu, hipMemcpyDeviceToHost));
+    for (auto &h : hc) h.tend = tend;
+    RCHK(c, hipMemcpy(c->clu, hc.data(), sizeof(RClu) * c->nclu, hipMemcpyHostToDevice));
+    if (use_graph && (!c->gexec || c->graph_steps != batch_steps)) {
+        if (c->gexec) RCHK(c, hipGraphExecDestroy(c->gexec));
+        if (c->graph) RCHK(c, hipGraphDestroy(c->graph));
+        c->gexec = nullptr;
+        c->graph = nullptr;
+        RCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e = hipSuccess;
+        for (int s = 0; s < batch_steps && e == hipSuccess; s++) e = r_step(c);
+        hipGraph_t gph = nullptr;
+        const hipError_t e2 = hipStreamEndCapture(c->stream, &gph);
+        if (e != hipSuccess) return rfail(c, HH_E_HIP, std::string("step capture: ") + hipGetErrorString(e));
+        if (e2 != hipSuccess) return rfail(c, HH_E_HIP, std::string("end capture: ") + hipGetErrorString(e2));
+        c->graph = gph;
+        RCHK(c, hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+        c->graph_steps = batch_steps;
+    }
+    RGlob g;
+    RCHK(c, hipEventRecord(c->ev0, c->stream));
+    for (;;) {
+        if (use_graph) {
+            RCHK(c, hipGraphLaunch(c->gexec, c->stream));
+        } else {
+            for (int s = 0; s < batch_steps; s++) {
+                const hipError_t e = r_step(c);
+                if (e != hipSuccess) return rfail(c, HH_E_HIP, std::string("step launch: ") + hipGetErrorString(e));
+            }
+        }
+        c->steps_launched += batch_steps;
+        RCHK(c, hipMemcpyAsync(&g, c->glob, sizeof(RGlob), hipMemcpyDeviceToHost, c->stream));
+        RCHK(c, hipStreamSynchronize(c->stream));
+        if (g.error == 1) return rfail(c, HH_E_STATE, "a cluster's block time stopped advancing (step-size collapse)");
+        if (g.error == 2) return rfail(c, HH_E_STATE, "a particle state became non-finite");
+        if (g.nactive == 0) break;
+    }
+    RCHK(c, hipEventRecord(c->ev1, c->stream));
+    RCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    RCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_run_ms = ms;
+    return HH_OK;
+}
+
+int hhr_stats(hhr_ctx *c, int k, long long *num_step, long long *num_bstep, double *tsys, long long *dtmin_hits)
+{
+    if (!c || k < 0 || k >= c->nclu) return HH_E_ARG;
+    RCHK(c, hipSetDevice(c->device));
+    RClu h;
+    RCHK(c, hipMemcpy(&h, c->clu + k, sizeof(RClu), hipMemcpyDeviceToHost));
+    if (num_step) *num_step = h.num_step;
+    if (num_bstep) *num_bstep = h.num_bstep;
+    if (tsys) *tsys = h.tsys;
+    if (dtmin_hits) *dtmin_hits = h.dtmin_hits;
+    return HH_OK;
+}
+
+double hhr_last_run_ms(const hhr_ctx *c) { return c ? c->last_run_ms : -1.0; }
+long long hhr_steps_launched(const hhr_ctx *c) { return c ? c->steps_launched : -1; }
+
+// Reference Particle records of cluster k.
+int hhr_get_particles(hhr_ctx *c, int k, double *recs)
+{
+    if (!c || !recs || k < 0 || k >= c->nclu) return HH_E_ARG;
+    RCHK(c, hipSetDevice(c->device));
+    const int o = c->off[k], nk = c->n[k], rw = c->rw, pw = p_words(c->order);
+    std::vector<double> w((size_t)rw * nk);
+    for (int q = 0; q < rw; q++)
+        RCHK(c, hipMemcpy(w.data() + (size_t)q * nk, c->rp + (size_t)q * c->npad + o, sizeof(double) * nk,
+                          hipMemcpyDeviceToHost));
+    for (int i = 0; i < nk; i++) {
+        double *p = recs + (size_t)i * pw;
+        std::memset(p, 0, sizeof(double) * pw);
+        auto g = [&](int q) { return w[(size_t)q * nk + i]; };
+        p[0] = g(R_ID);
+        p[1] = g(3);
+        p[2] = g(7);
+        p[3] = g(R_DT);
+        for (int d = 0; d < 3; d++) {
+            p[4 + d] = g(0 + d);
+            p[7 + d] = g(4 + d);
+            p[10 + d] = g(8 + d);
+            p[13 + d] = g(11 + d);
+            p[16 + d] = g(14 + d);
+            p[19 + d] = g(17 + d);
+            if (c->order >= 6) {
+                p[22 + d] = g(20 + d);
+                p[25 + d] = g(23 + d);
+            }
+            if (c->order == 8) {
+                p[28 + d] = g(R_D6A + d);
+                p[31 + d] = g(R_D7A + d);
+            }
+        }
+    }
+    return HH_OK;
+}
+
+// Total energy of cluster k (calc_energy, SRC/nbodysystem.h:633-650): the
+// potential kernel on the resident records, kinetic and potential sums in
+// particle order on the host.
+int hhr_energy(hhr_ctx *c, int k, double *energy)
+{
+    if (!c || !energy || k < 0 || k >= c->nclu) return HH_E_ARG;
+    RCHK(c, hipSetDevice(c->device));
+    const int o = c->off[k], nk = c->n[k];
+    // a private tile launch over this cluster's records
+    hh_ctx tmp;
+    tmp.order = c->order;
+    tmp.nbody = nk;
+    tmp.variant = c->variant;
+    tmp.device = c->device;
+    tmp.npad = c->npad;
+    tmp.stream = c->stream;
+    tmp.d_part = c->part;
+    tmp.part_cap = c->part_cap;
+    int r;
+    switch (c->variant) {
+    case HH_FP64: r = launch_tile<0, FP64, false>(&tmp, c->rp + o, nk, c->eps2[k], c->pot, nullptr); break;
+    case HH_SPLIT: r = launch_tile<0, SPLIT, false>(&tmp, c->rp + o, nk, c->eps2[k], c->pot, nullptr); break;
+    default: r = launch_tile<0, EXACT, false>(&tmp, c->rp + o, nk, c->eps2[k], c->pot, nullptr); break;
+    }
+    tmp.stream = nullptr;
+    tmp.d_part = nullptr;
+    if (r != HH_OK) return rfail(c, r, tmp.err);
+    std::vector<double> pot(nk), m(nk), v((size_t)3 * nk);
+    RCHK(c, hipMemcpyAsync(pot.data(), c->pot, sizeof(double) * nk, hipMemcpyDeviceToHost, c->stream));
+    RCHK(c, hipMemcpyAsync(m.data(), c->rp + (size_t)3 * c->npad + o, sizeof(double) * nk, hipMemcpyDeviceToHost, c->stream));
+    for (int d = 0; d < 3; d++)
+        RCHK(c, hipMemcpyAsync(v.data() + (size_t)d * nk, c->rp + (size_t)(4 + d) * c->npad + o, sizeof(double) * nk,
+                               hipMemcpyDeviceToHost, c->stream));
+    RCHK(c, hipStreamSynchronize(c->stream));
+    double ke = 0.0, pe = 0.0;
+    for (int i = 0; i < nk; i++) {
+        const double vx = v[i], vy = v[(size_t)nk + i], vz = v[(size_t)2 * nk + i];
+        ke += m[i] * (vx * vx + vy * vy + vz * vz);
+        pe += m[i] * pot[i];
+    }
+    ke *= 0.5;
+    pe *= 0.5;
+    *energy = ke + pe;
     return HH_OK;
 }
 

@@ -147,9 +147,7 @@ struct TileArgs {
     int ipad;                        // i-blocks * i per block
     double eps2;
     int pot_mask_r2;                 // potential self-exclusion by r2 > eps2 (order 4)
-    double *__restrict__ part;
-    unsigned *__restrict__ cnt;
-    int last_block;                  // nslice > 1: the last workgroup of an i-block sums the slices
+    double *__restrict__ part;       // slice partials part[(s*NC + c)*ipad + i] (nslice > 1)
     double *__restrict__ fout;       // [nact][F] (ORDER > 0) or [nact] (ORDER 0)
     double *__restrict__ pout;       // fused potential [nact] or null
 };
@@ -387,7 +385,6 @@ __global__ __launch_bounds__(NT) void tile_kernel(TileArgs a)
     // i words actually used: pos (+ vel, acc, jrk); mass not needed
     constexpr int QI = ORDER == 0 ? 3 : Q;
     __shared__ double sj[TJ * L];
-    __shared__ unsigned s_last;
 
     const int tid = threadIdx.x;
     const int iblock = blockIdx.x, slice = blockIdx.y;
@@ -484,54 +481,47 @@ __global__ __launch_bounds__(NT) void tile_kernel(TileArgs a)
         }
         return;
     }
+    // several slices: partial sums, added by slice_reduce_kernel.  No
+    // cross-workgroup handshake here: on gfx950 an agent-scope fence is an L2
+    // write-back plus invalidate, far dearer than the next kernel boundary.
     const size_t ipad = (size_t)a.ipad;
 #pragma unroll
     for (int k = 0; k < NI; k++)
 #pragma unroll
         for (int c = 0; c < NC; c++) a.part[((size_t)slice * NC + c) * ipad + ii[k]] = A[k][c];
-    if (!a.last_block) return;  // slice_reduce_kernel adds the slices
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) s_last = (atomicAdd(&a.cnt[iblock], 1u) == (unsigned)(a.nslice - 1));
-    __syncthreads();
-    if (!s_last) return;
-    // Acquire: the fence invalidates this CU's L1, so the plain loads below
-    // see every slice's partials (released by the writers' fence + atomic).
-    __threadfence();
-#pragma unroll
-    for (int k = 0; k < NI; k++) {
-        if (ii[k] >= a.nact) continue;
-        double sum[NC];
-#pragma unroll
-        for (int c = 0; c < NC; c++) sum[c] = 0.0;
-        for (int sl = 0; sl < a.nslice; sl++) {
-#pragma unroll
-            for (int c = 0; c < NC; c++) sum[c] += a.part[((size_t)sl * NC + c) * ipad + ii[k]];
-        }
-#pragma unroll
-        for (int c = 0; c < NC; c++) {
-            if (c < F) a.fout[(size_t)ii[k] * F + c] = sum[c];
-            else a.pout[ii[k]] = sum[c];
-        }
-    }
-    if (tid == 0) a.cnt[iblock] = 0u;
 }
 
-// Sum of the j-slice partials of tile_kernel in slice order (the same order
-// as its last-workgroup path), one thread per (component, i): the reduction
-// of SRC/hermite6-titan.cu:441-459 without its warp-shuffle tree, since
-// every thread here owns a whole sum.
+// Sum of the j-slice partials part[(s*nc + c)*ipad + i] over s, for i in
+// [0, ni), written as fout[i*f + c] (c < f) or pout[i] (c == f).  Each sum
+// is split over RL = 8 lanes (lane l takes s = l, l + 8, ...) and the lanes
+// are combined by a fixed xor-shuffle tree, so the order of additions, and
+// the result, depend only on nslice.  The reference reduces its NJBLOCK
+// partials with a warp-shuffle tree too (SRC/hermite6-titan.cu:441-459).
+constexpr int RL = 8;
+__device__ __forceinline__ double lane_tree8(double x)
+{
+    x += __shfl_xor(x, 4, 64);
+    x += __shfl_xor(x, 2, 64);
+    x += __shfl_xor(x, 1, 64);
+    return x;
+}
+
 __global__ __launch_bounds__(256) void slice_reduce_kernel(const double *__restrict__ part, int nslice, int nc,
-                                                           int f, int ipad, int nact, double *__restrict__ fout,
+                                                           int f, int ipad, int ni, double *__restrict__ fout,
                                                            double *__restrict__ pout)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nact * nc) return;
-    const int c = t / nact, i = t - c * nact;
+    const int t = blockIdx.x * (blockDim.x / RL) + threadIdx.x / RL;  // one (c, i) per RL lanes
+    const int l = threadIdx.x % RL;
+    const bool live = t < ni * nc;
+    const int c = live ? t / ni : 0, i = live ? t - c * ni : 0;
     double s = 0.0;
-    for (int sl = 0; sl < nslice; sl++) s += part[((size_t)sl * nc + c) * ipad + i];
-    if (c < f) fout[(size_t)i * f + c] = s;
-    else pout[i] = s;
+    if (live)
+        for (int sl = l; sl < nslice; sl += RL) s += part[((size_t)sl * nc + c) * ipad + i];
+    s = lane_tree8(s);
+    if (live && l == 0) {
+        if (c < f) fout[(size_t)i * f + c] = s;
+        else pout[i] = s;
+    }
 }
 
 // ---- j-parallel kernel for short active lists ----------------------------
@@ -540,9 +530,8 @@ __global__ __launch_bounds__(256) void slice_reduce_kernel(const double *__restr
 // (coalesced SoA loads straight from HBM, no LDS staging) and every lane
 // holds a group of GI i-particles (uniform across the workgroup, read with
 // scalar-friendly broadcast loads).  Per-lane sums are reduced across the
-// wave with xor-shuffles, across the 4 waves through LDS, and across the
-// j-slices by the last workgroup of each i-group in slice order
-// (deterministic).  FP64 and SPLIT bodies only: the EXACT variant must sum j
+// wave with xor-shuffles and across the 4 waves through LDS; the j-slices
+// are added by slice_reduce_kernel (deterministic).  FP64 and SPLIT bodies only: the EXACT variant must sum j
 // in index order and uses tile_kernel with one slice.
 struct JParArgs {
     const double *__restrict__ src;
@@ -552,8 +541,7 @@ struct JParArgs {
     int jchunk;      // j per workgroup, a multiple of NT
     int nslice;
     double eps2;
-    double *__restrict__ part;   // [(slice*ngroups + group)*V + v]
-    unsigned *__restrict__ cnt;  // per i-group arrival counters
+    double *__restrict__ part;   // part[(slice*NC + c)*ipad + i], ipad = ngroups*GI
     double *__restrict__ fout;
     double *__restrict__ pout;
 };
@@ -576,8 +564,6 @@ __global__ __launch_bounds__(NT) void jpar_kernel(JParArgs a)
     constexpr int NW = NT / 64;
     static_assert(V <= NT, "too many values per group");
     __shared__ double red[NW][V];
-    __shared__ double red2[NT];
-    __shared__ unsigned s_last;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int group = blockIdx.x, slice = blockIdx.y;
@@ -635,43 +621,16 @@ __global__ __launch_bounds__(NT) void jpar_kernel(JParArgs a)
         for (int w = 0; w < NW; w++) tot += red[w][tid];
     }
     const int gi = tid / NC, ci = tid - gi * NC;
+    if (tid >= V) return;
+    const int i = group * GI + gi;
     if (a.nslice == 1) {
-        if (tid < V) {
-            const int i = group * GI + gi;
-            if (i < a.nact) {
-                if (ci < F) a.fout[(size_t)i * F + ci] = tot;
-                else a.pout[i] = tot;
-            }
+        if (i < a.nact) {
+            if (ci < F) a.fout[(size_t)i * F + ci] = tot;
+            else a.pout[i] = tot;
         }
         return;
     }
-    if (tid < V) a.part[((size_t)slice * a.ngroups + group) * V + tid] = tot;
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) s_last = (atomicAdd(&a.cnt[group], 1u) == (unsigned)(a.nslice - 1));
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();  // acquire: see the other slices' partials
-    // last workgroup: sum the slices, NSUB interleaved partial sums per value
-    constexpr int NSUB = NT / V;
-    double s = 0.0;
-    if (tid < V * NSUB) {
-        const int v = tid / NSUB, sub = tid - v * NSUB;
-        for (int sl = sub; sl < a.nslice; sl += NSUB) s += a.part[((size_t)sl * a.ngroups + group) * V + v];
-    }
-    red2[tid] = s;
-    __syncthreads();
-    if (tid < V) {
-        double t = 0.0;
-#pragma unroll
-        for (int sub = 0; sub < NSUB; sub++) t += red2[tid * NSUB + sub];
-        const int i = group * GI + gi;
-        if (i < a.nact) {
-            if (ci < F) a.fout[(size_t)i * F + ci] = t;
-            else a.pout[i] = t;
-        }
-    }
-    if (tid == 0) a.cnt[group] = 0u;
+    a.part[((size_t)slice * NC + ci) * ((size_t)a.ngroups * GI) + i] = tot;
 }
 
 }  // namespace hh

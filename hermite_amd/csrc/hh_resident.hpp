@@ -1,0 +1,594 @@
+// GPU-resident block-timestep integration of one or many independent
+// clusters (ensembles): the whole block step of SRC/nbodysystem.h:692-765
+// runs on the device, so a step costs five kernel launches (capturable in a
+// hipGraph) and no host round trip.
+//
+// Per block step:
+//   r_min_kernel      per-cluster next block time min_i(tlast_i + dt_i)
+//   r_compact_kernel  active list of every cluster (wave-aggregated append)
+//   r_plan_kernel     per-cluster dtlim, step counters, kernel choice and
+//                     geometry, task table of the persistent force kernel
+//   r_predict_kernel  prediction of every particle of the active clusters
+//   r_force_kernel    persistent: workgroups pull (cluster, i-group, j-slice)
+//                     tasks from a device queue and write partial forces
+//   r_reduce_correct_kernel
+//                     sums each active particle's slices in a fixed order and
+//                     applies the Hermite corrector (SRC/hermite{4,6,8}.h
+//                     correct) to it in place.
+// Particles are not permuted (the reference sorts them by dt level,
+// SRC/nbodysystem.h:505-555): the active set is taken by comparison, and j
+// is always summed in the particles' original order.
+//
+// Resident record, SoA rp[w*npad + i]: words 0..25 are the j-store layout of
+// hh_kernels.hpp (pos mass vel tlast acc jrk snp crk d4a d5a), then dt (26),
+// id bits (27), d6a (28-30), d7a (31-33).
+#pragma once
+
+namespace hh {
+
+constexpr int R_DT = 26, R_ID = 27, R_D6A = 28, R_D7A = 31;
+__host__ __device__ constexpr int r_words(int o) { return o == 8 ? 34 : 28; }
+
+constexpr int R_GI = 4;          // i per group of the j-parallel task
+constexpr int R_JPAR_MAX = 64;   // active counts up to this use j-parallel tasks
+constexpr int R_SMAX = 64;       // most j-slices per i-group
+constexpr int R_TASK_TARGET = 2048;
+
+struct RClu {
+    double tsys, tend, dtmax, dtmin, eps2, etapow;
+    double tnext, dtlim;
+    unsigned long long tmin_bits;
+    long long num_step, num_bstep, dtmin_hits;
+    int off, n, ntiles;  // particles [off, off + n), tiles from off / TJ
+    int nact;
+    int mode;            // 0 idle, 1 i-parallel tiles, 2 j-parallel groups
+    int ngrp, nslice, tps, jchunk;
+    int task0, ntask, act0, ipad;
+    long long pofs;      // partials part[pofs + (s*NC + c)*ipad + ii]
+};
+
+struct RGlob {
+    int next_task, total_tasks, nclu, nactive, exact;
+    int error;     // a cluster's block time stopped advancing
+    int total_active;
+    int pad;
+};
+
+struct RArgs {
+    double *__restrict__ rp;     // resident records, SoA
+    double *__restrict__ jpr;    // predictions, SoA
+    double *__restrict__ tn;     // tlast + dt per slot (+inf on padding)
+    int *__restrict__ ilist;     // active list, at each cluster's offset
+    const int *__restrict__ tile_clu;
+    RClu *__restrict__ clu;
+    RGlob *__restrict__ glob;
+    double *__restrict__ part;
+    int npad;                    // SoA stride = total padded slots
+    int ntiles_total;
+};
+
+__device__ __forceinline__ double r_inf() { return __longlong_as_double(0x7ff0000000000000LL); }
+
+// ------------------------------------------------------------ corrector
+// Device restatement of the per-order Particle::correct with its quantised
+// Aarseth step (SRC/hermite4.h:118-153, SRC/hermite6.h:136-184,
+// SRC/hermite8.h:166-222), same operation order, no contraction.
+struct D3 {
+    double x, y, z;
+    __device__ D3() : x(0), y(0), z(0) {}
+    __device__ D3(double a, double b, double c) : x(a), y(b), z(c) {}
+    __device__ D3 operator+(const D3 &o) const { return D3(x + o.x, y + o.y, z + o.z); }
+    __device__ D3 operator-(const D3 &o) const { return D3(x - o.x, y - o.y, z - o.z); }
+    __device__ D3 operator-() const { return D3(-x, -y, -z); }
+    __device__ D3 operator*(double s) const { return D3(x * s, y * s, z * s); }
+    __device__ D3 operator/(double s) const { const double r = 1.0 / s; return *this * r; }
+    __device__ double norm2() const { return x * x + y * y + z * z; }
+};
+__device__ __forceinline__ D3 operator*(double s, const D3 &v) { return v * s; }
+
+template <int N>
+__device__ __forceinline__ double r_pow_nth_quant(double x)
+{
+    unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    unsigned long long e = b >> 52;
+    e = (e - (unsigned long long)(1023 % N)) / N + (unsigned long long)(1023 - 1023 / N);
+    return __longlong_as_double((long long)(e << 52));
+}
+
+struct RPart {  // one particle's words, loaded from / stored to the SoA record
+    double *rp;
+    int npad, i;
+    __device__ double &w(int k) const { return rp[(size_t)k * npad + i]; }
+    __device__ D3 v3(int k) const { return D3(w(k), w(k + 1), w(k + 2)); }
+    __device__ void set3(int k, const D3 &v) const { w(k) = v.x; w(k + 1) = v.y; w(k + 2) = v.z; }
+};
+
+// f: acc jrk [snp [crk]] of this particle.  Returns the new dt.
+template <int ORDER>
+__device__ void r_correct(const RPart &p, const double *f, double etapow, double dtlim)
+{
+    const double dt = p.w(R_DT);
+    const unsigned long long b = (unsigned long long)__double_as_longlong(dt) - (1ULL << 52);
+    const double h = __longlong_as_double((long long)b);
+    const double hinv = __longlong_as_double((long long)((2046ULL << 52) - b));
+    const D3 fa(f[0], f[1], f[2]), fj(f[3], f[4], f[5]);
+    const D3 acc = p.v3(8), jrk = p.v3(11), vel = p.v3(4);
+    const D3 Ap = fa + acc, Am = fa - acc;
+    const D3 Jp = (fj + jrk) * h, Jm = (fj - jrk) * h;
+    double u, l;
+    if constexpr (ORDER == 4) {
+        const D3 vel1 = vel + h * (Ap - (1. / 3.) * Jm);
+        const D3 pos = p.v3(0) + h * ((vel + vel1) + h * (-(1. / 3.) * Am));
+        p.set3(0, pos);
+        p.set3(4, vel1);
+        p.set3(8, fa);
+        p.set3(11, fj);
+        p.w(7) = p.w(7) + dt;
+        D3 snp = (0.5 * hinv * hinv) * Jm;
+        const D3 crk = (1.5 * hinv * hinv * hinv) * (Jp - Am);
+        snp = snp + h * crk;
+        p.set3(14, snp);
+        p.set3(17, crk);
+        const double s0 = fa.norm2(), s1 = fj.norm2(), s2 = snp.norm2(), s3 = crk.norm2();
+        u = sqrt(s0 * s2) + s1;
+        l = sqrt(s1 * s3) + s2;
+        const double dtq = r_pow_nth_quant<2>(etapow * (u / l));
+        p.w(R_DT) = dtq < dtlim ? dtq : dtlim;
+    } else if constexpr (ORDER == 6) {
+        const D3 fs(f[6], f[7], f[8]), snp = p.v3(14);
+        const D3 Sp = (fs + snp) * (h * h), Sm = (fs - snp) * (h * h);
+        const D3 vel1 = vel + h * (Ap - 0.4 * Jm + (1. / 15) * Sp);
+        const D3 pos = p.v3(0) + h * ((vel + vel1) + h * (-0.4 * Am + (1. / 15) * Jp));
+        p.set3(0, pos);
+        p.set3(4, vel1);
+        p.set3(8, fa);
+        p.set3(11, fj);
+        p.set3(14, fs);
+        p.w(7) = p.w(7) + dt;
+        const double hinv2 = hinv * hinv, hinv3 = hinv2 * hinv, hinv4 = hinv2 * hinv2, hinv5 = hinv2 * hinv3;
+        D3 crk = (0.75 * hinv3) * (-5. * Am + 5. * Jp - Sm);
+        D3 d4a = (1.5 * hinv4) * (-Jm + Sp);
+        const D3 d5a = (7.5 * hinv5) * (3 * Am - 3 * Jp + Sm);
+        crk = crk + h * (d4a + 0.5 * h * d5a);
+        d4a = d4a + h * d5a;
+        p.set3(17, crk);
+        p.set3(20, d4a);
+        p.set3(23, d5a);
+        const double s0 = fa.norm2(), s1 = fj.norm2(), s2 = fs.norm2();
+        const double s3 = crk.norm2(), s4 = d4a.norm2(), s5 = d5a.norm2();
+        u = sqrt(s0 * s2) + s1;
+        l = sqrt(s3 * s5) + s4;
+        const double dtq = r_pow_nth_quant<6>(etapow * (u / l));
+        p.w(R_DT) = dtq < dtlim ? dtq : dtlim;
+    } else {
+        const D3 fs(f[6], f[7], f[8]), fc(f[9], f[10], f[11]), snp = p.v3(14), crk = p.v3(17);
+        const D3 Sp = (fs + snp) * (h * h), Sm = (fs - snp) * (h * h);
+        const D3 Cp = (fc + crk) * (h * h * h), Cm = (fc - crk) * (h * h * h);
+        const D3 vel1 = vel + h * (Ap + (-3. / 7.) * Jm + (2. / 21) * Sp - (1. / 105.) * Cm);
+        const D3 pos = p.v3(0) + h * ((vel + vel1) + h * ((-3. / 7.) * Am + (2. / 21) * Jp - (1. / 105.) * Sm));
+        p.set3(0, pos);
+        p.set3(4, vel1);
+        p.set3(8, fa);
+        p.set3(11, fj);
+        p.set3(14, fs);
+        p.set3(17, fc);
+        p.w(7) = p.w(7) + dt;
+        const double hinv2 = hinv * hinv, hinv3 = hinv2 * hinv, hinv4 = hinv2 * hinv2;
+        const double hinv5 = hinv2 * hinv3, hinv6 = hinv3 * hinv3, hinv7 = hinv3 * hinv4;
+        D3 d4a = (hinv4 * 24. / 32.) * (-5. * Jm + 5. * Sp - Cm);
+        D3 d5a = (hinv5 * 120. / 32.) * (21. * Am - 21. * Jp + 8. * Sm - Cp);
+        D3 d6a = (hinv6 * 720. / 32.) * (Jm - Sp + Cm / 3.);
+        const D3 d7a = (hinv7 * 5040. / 32.) * (-5. * Am + 5. * Jp - 2. * Sm + Cp / 3.);
+        const double h2 = 0.5 * h, h3 = (1. / 3.) * h;
+        d4a = d4a + h * (d5a + h2 * (d6a + h3 * d7a));
+        d5a = d5a + h * (d6a + h2 * d7a);
+        d6a = d6a + h * d7a;
+        p.set3(20, d4a);
+        p.set3(23, d5a);
+        p.set3(R_D6A, d6a);
+        p.set3(R_D7A, d7a);
+        const double s0 = fa.norm2(), s1 = fj.norm2(), s2 = fs.norm2();
+        const double s5 = d5a.norm2(), s6 = d6a.norm2(), s7 = d7a.norm2();
+        u = sqrt(s0 * s2) + s1;
+        l = sqrt(s5 * s7) + s6;
+        const double dtq = r_pow_nth_quant<10>(etapow * (u / l));
+        p.w(R_DT) = dtq < dtlim ? dtq : dtlim;
+    }
+}
+
+// Correct particle i with force f, clamp dt to dtmin, refresh tn[i].
+template <int ORDER>
+__device__ void r_finish(const RArgs &a, const RClu &c, int i, const double *f)
+{
+    const RPart p{a.rp, a.npad, i};
+    r_correct<ORDER>(p, f, c.etapow, c.dtlim);
+    double dt = p.w(R_DT);
+    if (c.dtmin > 0.0 && dt < c.dtmin) {
+        dt = c.dtmin;
+        p.w(R_DT) = dt;
+        atomicAdd((unsigned long long *)&a.clu[a.tile_clu[i / TJ]].dtmin_hits, 1ULL);
+    }
+    a.tn[i] = p.w(7) + dt;
+    // a non-finite state (a step that ran away on noisy derivatives) stops
+    // the run instead of integrating garbage
+    if (!isfinite(p.w(0) + p.w(1) + p.w(2) + p.w(4) + p.w(5) + p.w(6))) atomicExch(&a.glob->error, 2);
+}
+
+// ------------------------------------------------------------ schedule
+__device__ __forceinline__ double r_wave_min(double x)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) x = fmin(x, __shfl_xor(x, off, 64));
+    return x;
+}
+
+// One workgroup per 256-slot tile; a tile belongs to one cluster.
+__global__ __launch_bounds__(NT) void r_min_kernel(RArgs a)
+{
+    __shared__ double wmin[NT / 64];
+    const int tile = blockIdx.x, tid = threadIdx.x;
+    const int k = a.tile_clu[tile];
+    const RClu &c = a.clu[k];
+    if (tile * TJ == c.off && tid == 0) a.clu[k].nact = 0;  // first tile of the cluster
+    double t = a.tn[(size_t)tile * TJ + tid];
+    t = r_wave_min(t);
+    if ((tid & 63) == 0) wmin[tid >> 6] = t;
+    __syncthreads();
+    if (tid == 0) {
+        double m = wmin[0];
+        for (int w = 1; w < NT / 64; w++) m = fmin(m, wmin[w]);
+        // positive doubles order like their bit patterns
+        atomicMin(&a.clu[k].tmin_bits, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+__global__ __launch_bounds__(NT) void r_compact_kernel(RArgs a)
+{
+    const int tile = blockIdx.x, tid = threadIdx.x;
+    const int k = a.tile_clu[tile];
+    RClu &c = a.clu[k];
+    const double tmin = __longlong_as_double((long long)c.tmin_bits);
+    const int i = tile * TJ + tid;
+    const bool act = (tmin <= c.tend) && a.tn[i] == tmin;
+    const unsigned long long mask = __ballot(act);
+    if (mask == 0) return;
+    const int lane = tid & 63;
+    const int leader = __ffsll((long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&c.nact, __popcll(mask));
+    base = __shfl(base, leader, 64);
+    if (act) {
+        const unsigned long long below = lane == 0 ? 0ULL : (mask & ((~0ULL) >> (64 - lane)));
+        a.ilist[c.off + base + __popcll(below)] = i;
+    }
+}
+
+// calc_dtlim (SRC/nbodysystem.h:652-661) with floor() for the int cast;
+// 0 when tnext is off the power-of-two grid.
+__device__ double r_dtlim(double tnext, double dtmax)
+{
+    double dtlim = dtmax, s = tnext / dtmax;
+    for (int k = 0; k < 1100 && isfinite(s); k++) {
+        if (s == floor(s)) return dtlim;
+        s *= 2.0;
+        dtlim *= 0.5;
+    }
+    return 0.0;
+}
+
+template <int ORDER, bool POT>
+__host__ __device__ constexpr int r_nc() { return n_acc(ORDER, POT); }
+
+// One workgroup.  Geometry per cluster from its active count; the task
+// budget is shared in proportion to each cluster's nact * n work.
+template <int ORDER>
+__global__ __launch_bounds__(NT) void r_plan_kernel(RArgs a)
+{
+    constexpr int NC = f_words(ORDER);
+    __shared__ double s_work[NT];
+    __shared__ double s_total;
+    const int tid = threadIdx.x;
+    const int nclu = a.glob->nclu;
+    double w = 0.0;
+    for (int k = tid; k < nclu; k += NT) {
+        RClu &c = a.clu[k];
+        c.mode = 0;
+        c.ntask = 0;
+        if (c.nact > 0) w += (double)c.nact * c.n;
+    }
+    s_work[tid] = w;
+    __syncthreads();
+    if (tid == 0) {
+        double t = 0.0;
+        for (int q = 0; q < NT; q++) t += s_work[q];
+        s_total = t;
+    }
+    __syncthreads();
+    const double total = s_total;
+    for (int k = tid; k < nclu; k += NT) {
+        RClu &c = a.clu[k];
+        const double tmin = __longlong_as_double((long long)c.tmin_bits);
+        c.tmin_bits = (unsigned long long)__double_as_longlong(r_inf());
+        if (c.nact == 0) continue;
+        const double dtlim = r_dtlim(tmin, c.dtmax);
+        if (!(tmin > c.tsys) || dtlim <= 0.0) {  // collapsed step: stop this cluster
+            c.nact = 0;
+            c.tend = c.tsys;
+            atomicExch(&a.glob->error, 1);
+            continue;
+        }
+        c.tnext = tmin;
+        c.dtlim = dtlim;
+        c.num_step += c.nact;
+        c.num_bstep += 1;
+        c.tsys = tmin;
+        const int budget = max(1, (int)ceil(R_TASK_TARGET * ((double)c.nact * c.n) / total));
+        if (a.glob->exact || c.nact > R_JPAR_MAX) {
+            c.mode = 1;
+            c.ngrp = (c.nact + NT - 1) / NT;
+            int ns = a.glob->exact ? 1 : min(min(c.ntiles, R_SMAX), max(1, (budget + c.ngrp - 1) / c.ngrp));
+            c.tps = (c.ntiles + ns - 1) / ns;
+            c.nslice = (c.ntiles + c.tps - 1) / c.tps;
+        } else {
+            c.mode = 2;
+            c.ngrp = (c.nact + R_GI - 1) / R_GI;
+            int ns = min(c.ntiles, max(1, (budget + c.ngrp - 1) / c.ngrp));
+            const int tpc = (c.ntiles + ns - 1) / ns;  // tiles per chunk
+            c.jchunk = tpc * TJ;
+            c.nslice = (c.ntiles + tpc - 1) / tpc;
+        }
+        c.ntask = c.ngrp * c.nslice;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int t0 = 0, act = 0, nactive = 0;
+        long long p0 = 0;
+        for (int k = 0; k < nclu; k++) {
+            RClu &c = a.clu[k];
+            c.task0 = t0;
+            c.act0 = act;
+            c.pofs = p0;
+            t0 += c.ntask;
+            if (c.mode != 0) {
+                nactive++;
+                act += c.nact;
+                c.ipad = c.ngrp * (c.mode == 1 ? NT : R_GI);
+                p0 += (long long)c.nslice * NC * c.ipad;
+            }
+        }
+        a.glob->total_tasks = t0;
+        a.glob->next_task = 0;
+        a.glob->nactive = nactive;
+        a.glob->total_active = act;
+    }
+}
+
+// Predict every slot of every active cluster to its tnext.
+template <int ORDER, bool EX>
+__global__ __launch_bounds__(NT) void r_predict_kernel(RArgs a)
+{
+    constexpr int W = jp_words(ORDER), Q = pr_words(ORDER);
+    const int tile = blockIdx.x;
+    const RClu &c = a.clu[a.tile_clu[tile]];
+    if (c.mode == 0) return;
+    const int i = tile * TJ + threadIdx.x;
+    double r[W], q[Q];
+#pragma unroll
+    for (int w = 0; w < W; w++) r[w] = a.rp[(size_t)w * a.npad + i];
+    predict_words<ORDER, EX>(r, c.tnext, q);
+#pragma unroll
+    for (int w = 0; w < Q; w++) a.jpr[(size_t)w * a.npad + i] = q[w];
+}
+
+// Cluster of a task: the last cluster whose first task is <= task.
+__device__ int r_find_cluster(const RClu *clu, int nclu, int task)
+{
+    int lo = 0, hi = nclu - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (clu[mid].task0 <= task) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Persistent force + correct kernel.
+template <int ORDER, int MODE>
+__global__ __launch_bounds__(NT) void r_force_kernel(RArgs a)
+{
+    constexpr int Q = pr_words(ORDER);
+    constexpr int L = lds_words(ORDER);
+    constexpr int F = f_words(ORDER);
+    constexpr int NC = F;
+    constexpr int V = R_GI * NC;
+    constexpr int NW = NT / 64;
+    constexpr int FOLD = 8;
+    __shared__ double sj[TJ * L];
+    __shared__ double red[NW][V];
+    __shared__ int s_task;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int npad = a.npad;
+    const int nclu = a.glob->nclu;
+    const int total = a.glob->total_tasks;
+
+    for (;;) {
+        __syncthreads();
+        if (tid == 0) s_task = atomicAdd(&a.glob->next_task, 1);
+        __syncthreads();
+        const int task = s_task;
+        if (task >= total) break;
+        const int k = r_find_cluster(a.clu, nclu, task);
+        const RClu &c = a.clu[k];
+        const int local = task - c.task0;
+        const int grp = local / c.nslice, slice = local - grp * c.nslice;
+        const double eps2 = c.eps2;
+        const float eps2f = (float)eps2;
+        const double *src = a.jpr;
+
+        if (c.mode == 1) {
+            // ---- i in lanes, j-tiles of the slice through LDS
+            const int ii = grp * NT + tid;
+            const int gi = a.ilist[c.off + min(ii, c.nact - 1)];
+            double pi[Q];
+#pragma unroll
+            for (int w = 0; w < Q; w++) pi[w] = src[(size_t)w * npad + gi];
+            double A[NC];
+#pragma unroll
+            for (int q = 0; q < NC; q++) A[q] = 0.0;
+            const int tb = c.off / TJ;
+            const int t0 = slice * c.tps, t1 = min(t0 + c.tps, c.ntiles);
+            double jr[Q];
+#pragma unroll
+            for (int w = 0; w < Q; w++) jr[w] = src[(size_t)w * npad + (tb + t0) * TJ + tid];
+            for (int t = t0; t < t1; t++) {
+                __syncthreads();
+#pragma unroll
+                for (int w = 0; w < Q; w++) sj[tid * L + w] = jr[w];
+                __syncthreads();
+                if (t + 1 < t1) {
+#pragma unroll
+                    for (int w = 0; w < Q; w++) jr[w] = src[(size_t)w * npad + (tb + t + 1) * TJ + tid];
+                }
+                const int jbase = c.off + t * TJ;
+                if constexpr (MODE == EXACT) {
+                    const int jn = min(TJ, c.off + c.n - jbase);
+                    for (int jj = 0; jj < jn; jj++) {
+                        double q[L];
+#pragma unroll
+                        for (int w = 0; w < L; w++) q[w] = sj[jj * L + w];
+                        body_exact<ORDER, false>(pi, q, eps2, false, 0, A);
+                    }
+                } else if constexpr (MODE == FP64) {
+#pragma unroll 2
+                    for (int jj = 0; jj < TJ; jj++) {
+                        double q[L];
+#pragma unroll
+                        for (int w = 0; w < L; w++) q[w] = sj[jj * L + w];
+                        body_fp64<ORDER, false>(pi, q, eps2, false, 0, A);
+                    }
+                } else {
+                    for (int jb = 0; jb < TJ; jb += FOLD) {
+                        float Af[NC];
+#pragma unroll
+                        for (int q = 0; q < NC; q++) Af[q] = 0.0f;
+#pragma unroll 2
+                        for (int jj = jb; jj < jb + FOLD; jj++) {
+                            double q[L];
+#pragma unroll
+                            for (int w = 0; w < L; w++) q[w] = sj[jj * L + w];
+                            body_split<ORDER, false>(pi, q, eps2f, false, 0, Af);
+                        }
+#pragma unroll
+                        for (int q = 0; q < NC; q++) A[q] += (double)Af[q];
+                    }
+                }
+            }
+            double *part = a.part + c.pofs;
+#pragma unroll
+            for (int q = 0; q < NC; q++) part[((size_t)slice * NC + q) * c.ipad + ii] = A[q];
+        } else {
+            // ---- j in lanes, R_GI i-particles per group
+            if constexpr (MODE == EXACT) continue;  // EXACT always plans mode 1
+            int gi[R_GI];
+            double pi[R_GI][Q];
+#pragma unroll
+            for (int g = 0; g < R_GI; g++) {
+                gi[g] = a.ilist[c.off + min(grp * R_GI + g, c.nact - 1)];
+#pragma unroll
+                for (int w = 0; w < Q; w++) pi[g][w] = src[(size_t)w * npad + gi[g]];
+            }
+            double A[R_GI][NC];
+#pragma unroll
+            for (int g = 0; g < R_GI; g++)
+#pragma unroll
+                for (int q = 0; q < NC; q++) A[g][q] = 0.0;
+            const int j0 = c.off + slice * c.jchunk;
+            const int j1 = min(j0 + c.jchunk, c.off + c.ntiles * TJ);
+            for (int j = j0 + tid; j < j1; j += NT) {
+                double q[Q];
+#pragma unroll
+                for (int w = 0; w < Q; w++) q[w] = src[(size_t)w * npad + j];
+#pragma unroll
+                for (int g = 0; g < R_GI; g++) {
+                    if constexpr (MODE == FP64) {
+                        body_fp64<ORDER, false>(pi[g], q, eps2, false, 0, A[g]);
+                    } else {
+                        float Af[NC];
+#pragma unroll
+                        for (int qq = 0; qq < NC; qq++) Af[qq] = 0.0f;
+                        body_split<ORDER, false>(pi[g], q, eps2f, false, 0, Af);
+#pragma unroll
+                        for (int qq = 0; qq < NC; qq++) A[g][qq] += (double)Af[qq];
+                    }
+                }
+            }
+            __syncthreads();  // red reuse across tasks
+#pragma unroll
+            for (int g = 0; g < R_GI; g++)
+#pragma unroll
+                for (int q = 0; q < NC; q++) {
+                    const double x = wave_sum(A[g][q]);
+                    if (lane == 0) red[wave][g * NC + q] = x;
+                }
+            __syncthreads();
+            double tot = 0.0;
+            if (tid < V) {
+#pragma unroll
+                for (int w = 0; w < NW; w++) tot += red[w][tid];
+            }
+            if (tid < V) {
+                const int g = tid / NC, q = tid - g * NC;
+                a.part[c.pofs + ((size_t)slice * NC + q) * c.ipad + grp * R_GI + g] = tot;
+            }
+        }
+    }
+}
+
+// Cluster of an active-list position: the last cluster whose act0 <= pos
+// (idle clusters share act0 with the next active one, so they are never
+// the last such cluster for a valid position).
+__device__ int r_find_active(const RClu *clu, int nclu, int pos)
+{
+    int lo = 0, hi = nclu - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (clu[mid].act0 <= pos) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Sum every active particle's slice partials (RL lanes per particle, fixed
+// xor tree) and apply the corrector.  Grid-stride over the active list of
+// all clusters, so the launch shape is fixed (graph-capturable).
+template <int ORDER>
+__global__ __launch_bounds__(NT) void r_reduce_correct_kernel(RArgs a)
+{
+    constexpr int NC = f_words(ORDER);
+    constexpr int PB = NT / RL;  // particles per workgroup pass
+    const int nclu = a.glob->nclu, total = a.glob->total_active;
+    const int l = threadIdx.x % RL;
+    for (int base = blockIdx.x * PB; base < total; base += gridDim.x * PB) {
+        const int pos = base + threadIdx.x / RL;
+        const bool live = pos < total;
+        const int k = live ? r_find_active(a.clu, nclu, pos) : 0;
+        const RClu &c = a.clu[k];
+        const int ii = pos - c.act0;
+        double S[NC];
+#pragma unroll
+        for (int q = 0; q < NC; q++) S[q] = 0.0;
+        if (live) {
+            const double *part = a.part + c.pofs;
+            for (int sl = l; sl < c.nslice; sl += RL) {
+#pragma unroll
+                for (int q = 0; q < NC; q++) S[q] += part[((size_t)sl * NC + q) * c.ipad + ii];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NC; q++) S[q] = lane_tree8(S[q]);
+        if (live && l == 0) r_finish<ORDER>(a, c, a.ilist[c.off + ii], S);
+    }
+}
+
+}  // namespace hh

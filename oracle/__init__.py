@@ -80,6 +80,7 @@ def lib():
         L.orc_sys_integrate_one_block.argtypes = [C.c_void_p]
         L.orc_sys_integrate_one_block.restype = C.c_int
         L.orc_sys_integrate_one_dtmax.argtypes = [C.c_void_p]
+        L.orc_sys_integrate_until_inplace.argtypes = [C.c_void_p, C.c_double]
         for name in ("orc_sys_energy", "orc_sys_tsys", "orc_sys_init_energy"):
             getattr(L, name).restype = C.c_double
             getattr(L, name).argtypes = [C.c_void_p]
@@ -203,6 +204,11 @@ class System:
 
     def integrate_one_dtmax(self):
         lib().orc_sys_integrate_one_dtmax(self.h)
+
+    def integrate_until_inplace(self, tend):
+        """Block steps without the dt-level permutation (the GPU-resident
+        driver's schedule) until tsys reaches tend."""
+        lib().orc_sys_integrate_until_inplace(self.h, tend)
 
     def energy(self) -> float:
         return lib().orc_sys_energy(self.h)

@@ -582,6 +582,43 @@ int orc_sys_integrate_one_block(orc_sys *s)
     return nact;
 }
 
+/* Block step without the dt-level permutation: the active set is every
+ * particle with tlast + dt == min_i(tlast_i + dt_i), taken in index order,
+ * and particles stay in place, so j is always summed in the original index
+ * order.  Same physics as orc_sys_integrate_one_block (corrector, quantised
+ * step, dtlim); this is the schedule the GPU-resident driver implements.
+ * Returns nact. */
+int orc_sys_integrate_one_block_inplace(orc_sys *s, int *ilist)
+{
+    double tnext = s->ptcl[W_TLAST] + s->ptcl[W_DT];
+    for (int i = 1; i < s->n; i++) {
+        const double *p = s->ptcl + (size_t)i * s->ps;
+        const double t = p[W_TLAST] + p[W_DT];
+        if (t < tnext) tnext = t;
+    }
+    const double dtlim = orc_calc_dtlim(tnext, s->dtmax);
+    int nact = 0;
+    for (int i = 0; i < s->n; i++) {
+        const double *p = s->ptcl + (size_t)i * s->ps;
+        if (p[W_TLAST] + p[W_DT] == tnext) ilist[nact++] = i;
+    }
+    orc_predict(s->order, s->n, s->ptcl, tnext, s->pred);
+    orc_force_list(s->order, s->n, s->pred, nact, ilist, s->eps2, s->force);
+    for (int k = 0; k < nact; k++)
+        orc_correct(s->order, s->ptcl + (size_t)ilist[k] * s->ps, s->force + (size_t)k * s->fs, s->etapow, dtlim);
+    s->num_step += nact;
+    s->num_bstep++;
+    s->tsys = tnext;
+    return nact;
+}
+
+void orc_sys_integrate_until_inplace(orc_sys *s, double tend)
+{
+    int *ilist = (int *)malloc(sizeof(int) * (size_t)s->n);
+    while (s->tsys < tend) orc_sys_integrate_one_block_inplace(s, ilist);
+    free(ilist);
+}
+
 /* integrate_one_dtmax: SRC/nbodysystem.h:904-914 */
 void orc_sys_integrate_one_dtmax(orc_sys *s)
 {

@@ -1,0 +1,154 @@
+"""GPU-resident block steps (hhr_*): schedule, prediction, force and
+corrector on the device, one or many clusters, hipGraph replay.
+
+The resident driver keeps particles in place (no dt-level permutation), so
+its EXACT variant is bit-identical to the oracle's in-place schedule
+(orc_sys_integrate_one_block_inplace); that schedule reproduces the nact
+known answers of the reference logs.  An ensemble of independent clusters
+must give each cluster exactly what it gives when run alone."""
+import numpy as np
+import pytest
+
+import hermite_amd as H
+import oracle as O
+from tests.test_golden import NACT_KAT
+
+pytestmark = pytest.mark.gpu
+
+ORDERS = (4, 6, 8)
+
+
+def initialised(order, n, seed, eps2, eta, eta_s, dtmax, rmax=22.8):
+    """Particles after the reference init_step (oracle driver)."""
+    m, x, v = H.plummer(n, seed, rmax)
+    s = O.System(order, O.make_particles(order, m, x, v), 0.0, eps2, eta, eta_s, dtmax)
+    s.init_step()
+    return s
+
+
+@pytest.mark.parametrize("order", ORDERS)
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_resident_exact_bitwise_vs_oracle_inplace(order, use_graph):
+    n, eps2 = 500, (1 / 32) ** 2
+    eta, eta_s = {4: (0.1, 0.01), 6: (0.4, 0.1), 8: (0.6, 0.1)}[order]
+    o = initialised(order, n, 41, eps2, eta, eta_s, 1 / 16)
+    r = H.Resident(order, "exact", [(o.ptcl(), eps2, eta, 1 / 16, 0.0, 0.0)])
+    print(f"resident exact order={order} graph={use_graph}", flush=True)
+    for t in (1 / 16, 1 / 4):
+        o.integrate_until_inplace(t)
+        r.run(t, use_graph=use_graph, batch=4)
+        st = r.stats()
+        assert (st["num_step"], st["num_bstep"], st["tsys"]) == (o.num_step, o.num_bstep, o.tsys)
+        assert np.array_equal(r.particles(), o.ptcl())
+    assert r.energy() == o.energy()
+    r.close()
+
+
+@pytest.mark.parametrize("order", ORDERS)
+def test_resident_exact_reproduces_reference_nact(order):
+    eta, eta_s, want1, want2 = NACT_KAT[order]
+    n = 1024
+    eps2 = (4.0 / n) ** 2
+    o = initialised(order, n, 19640916, eps2, eta, eta_s, 1 / 16, rmax=100.0)
+    r = H.Resident(order, "exact", [(o.ptcl(), eps2, eta, 1 / 16, 0.0, 0.0)])
+    r.run(1.0)
+    s1 = r.stats()
+    r.run(2.0)
+    s2 = r.stats()
+    r.close()
+    assert "%e" % (s1["num_step"] / s1["num_bstep"]) == want1
+    assert "%e" % ((s2["num_step"] - s1["num_step"]) / (s2["num_bstep"] - s1["num_bstep"])) == want2
+
+
+@pytest.mark.parametrize("order", ORDERS)
+def test_resident_fp64_statistics(order):
+    """FP64 kernel: same mean nact as the reference logs, |dE/E| of the same
+    size as the oracle's on the same schedule."""
+    eta, eta_s, want1, _ = NACT_KAT[order]
+    n = 1024
+    eps2 = (4.0 / n) ** 2
+    o = initialised(order, n, 19640916, eps2, eta, eta_s, 1 / 16, rmax=100.0)
+    P0, e0 = o.ptcl(), o.init_energy
+    o.integrate_until_inplace(1.0)
+    de_o = abs((o.energy() - e0) / e0)
+    r = H.Resident(order, "fp64", [(P0, eps2, eta, 1 / 16, 0.0, 0.0)])
+    r.run(1.0)
+    st = r.stats()
+    de = abs((r.energy() - e0) / e0)
+    print(f"order={order} resident fp64: nact={st['num_step'] / st['num_bstep']:.6e} (reference {want1}) "
+          f"|dE/E|={de:.3e} (oracle {de_o:.3e})", flush=True)
+    assert abs(st["num_step"] / st["num_bstep"] / float(want1) - 1) < 0.01
+    assert de < 3 * de_o + 1e-13
+    r.close()
+
+
+def test_resident_split_behaviour():
+    """The FP32 interaction body leaves ~1e-7 relative noise in each force
+    (the split parity tests bound it).  The 4th-order scheme integrates with
+    it, at a larger energy error; the 8th-order derivative reconstruction
+    (d4a..d7a ~ force differences / h^4..h^7) amplifies it until the state
+    runs away, which the driver reports instead of integrating garbage."""
+    n = 1024
+    m, x, v = H.plummer(n, 19640916, 100.0)
+    eps2 = (4.0 / n) ** 2
+    res = {}
+    for order in (4, 6):
+        eta, eta_s, want1, _ = NACT_KAT[order]
+        o = initialised(order, n, 19640916, eps2, eta, eta_s, 1 / 16, rmax=100.0)
+        r = H.Resident(order, "split", [(o.ptcl(), eps2, eta, 1 / 16, 0.0, 0.0)])
+        r.run(1.0)
+        st = r.stats()
+        res[order] = abs((r.energy() - o.init_energy) / o.init_energy)
+        print(f"split order={order}: nact={st['num_step'] / st['num_bstep']:.4e} |dE/E|={res[order]:.3e}", flush=True)
+        assert abs(st["num_step"] / st["num_bstep"] / float(want1) - 1) < 0.01
+        assert res[order] < 1e-6
+        r.close()
+    eta, eta_s, _, _ = NACT_KAT[8]
+    o = initialised(8, n, 19640916, eps2, eta, eta_s, 1 / 16, rmax=100.0)
+    r = H.Resident(8, "split", [(o.ptcl(), eps2, eta, 1 / 16, 0.0, 0.0)])
+    with pytest.raises(H.HHError, match="non-finite|stopped advancing"):
+        r.run(1.0)
+    r.close()
+
+
+def test_ensemble_equals_clusters_run_alone():
+    """Uneven clusters in one resident context: every cluster's particles,
+    step counts and energy equal those of the same cluster run alone
+    (EXACT variant: no summation-order freedom)."""
+    order, eta, eta_s, dtmax = 4, 0.1, 0.01, 1 / 16
+    sizes = [256, 300, 700, 1024, 333, 512, 1500, 260]
+    clusters, solo = [], []
+    for k, n in enumerate(sizes):
+        eps2 = (4.0 / n) ** 2
+        o = initialised(order, n, 100 + k, eps2, eta, eta_s, dtmax)
+        clusters.append((o.ptcl(), eps2, eta, dtmax, 0.0, 0.0))
+    ens = H.Resident(order, "exact", clusters)
+    ens.run(0.25)
+    for k, c in enumerate(clusters):
+        alone = H.Resident(order, "exact", [c])
+        alone.run(0.25)
+        assert ens.stats(k) == alone.stats(0), k
+        assert np.array_equal(ens.particles(k), alone.particles(0)), k
+        assert ens.energy(k) == alone.energy(0)
+        alone.close()
+    ens.close()
+
+
+def test_ensemble_fp64_matches_oracle_statistically():
+    order, eta, eta_s, dtmax = 4, 0.1, 0.01, 1 / 16
+    sizes = [256, 1024, 4096, 512]
+    clusters, oracles = [], []
+    for k, n in enumerate(sizes):
+        eps2 = (4.0 / n) ** 2
+        o = initialised(order, n, 200 + k, eps2, eta, eta_s, dtmax)
+        clusters.append((o.ptcl(), eps2, eta, dtmax, 0.0, 0.0))
+        oracles.append(o)
+    ens = H.Resident(order, "fp64", clusters)
+    ens.run(0.25)
+    for k, o in enumerate(oracles):
+        e0 = o.init_energy
+        o.integrate_until_inplace(0.25)
+        st = ens.stats(k)
+        assert abs(st["num_step"] / o.num_step - 1) < 0.02
+        assert abs((ens.energy(k) - e0) / e0) < 3 * abs((o.energy() - e0) / e0) + 1e-12
+    ens.close()
