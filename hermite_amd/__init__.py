@@ -57,6 +57,9 @@ def hip_lib():
         L.hh_last_error.restype = C.c_char_p
         L.hh_set_tuning.argtypes = [vp, C.c_int, C.c_int]
         L.hh_set_tuning_ex.argtypes = [vp, C.c_int, C.c_int]
+        ip = C.POINTER(C.c_int)
+        L.hh_set_tuning_table.argtypes = [vp, C.c_int, ip, ip, ip, ip]
+        L.hh_load_tuning_file.argtypes = [vp, C.c_char_p]
         L.hh_time_force.argtypes = [vp, C.c_int, C.c_double, C.c_int, _dp]
         L.hh_time_predict.argtypes = [vp, C.c_double, C.c_int, _dp]
         L.hh_force_async.argtypes = [vp, C.c_int, C.c_double, C.c_int]
@@ -76,6 +79,9 @@ def hip_lib():
         L.hhr_steps_launched.restype = C.c_longlong
         L.hhr_get_particles.argtypes = [vp, C.c_int, _dp]
         L.hhr_energy.argtypes = [vp, C.c_int, _dp]
+        L.hhr_set_timing.argtypes = [vp, C.c_int]
+        L.hhr_set_plan.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.hhr_force_timing.argtypes = [vp, _dp, C.POINTER(C.c_longlong)]
         _hip = L
     return _hip
 
@@ -94,6 +100,8 @@ def host_lib():
         L.hhd_create.argtypes = [C.c_int, C.c_int, _dp] + [C.c_double] * 6 + [C.c_int, C.c_int]
         L.hhd_destroy.argtypes = [vp]
         L.hhd_status.argtypes = [vp]
+        L.hhd_gravity.argtypes = [vp]
+        L.hhd_gravity.restype = vp
         L.hhd_error.argtypes = [vp]
         L.hhd_error.restype = C.c_char_p
         for name in ("hhd_init_step", "hhd_integrate_one_block", "hhd_integrate_one_dtmax"):
@@ -147,9 +155,13 @@ def make_particles(order: int, mass, pos, vel, ids=None, tlast: float = 0.0) -> 
 
 # ------------------------------------------------------------------ Gravity
 class Gravity:
-    """ctypes mirror of the reference's ``struct Gravity`` on the HIP back end."""
+    """ctypes mirror of the reference's ``struct Gravity`` on the HIP back end.
 
-    def __init__(self, order: int, nbody: int, variant: str = "fp64", device: int = 0):
+    tuning: True loads the autotuned launch table for the nearest tuned N of
+    this order and variant (hermite_amd/tuning/mi355x.json), False keeps the
+    built-in geometry choice."""
+
+    def __init__(self, order: int, nbody: int, variant: str = "fp64", device: int = 0, tuning: bool = True):
         L = hip_lib()
         self.order, self.nbody, self.variant = order, nbody, variant
         h = C.c_void_p()
@@ -158,6 +170,13 @@ class Gravity:
             raise HHError(f"hh_create({order}, {nbody}, {variant}) failed ({rc}): "
                           f"{L.hh_last_error(None).decode()}")
         self.h = h
+        self.tuning_rows = None
+        if tuning and variant != "exact":
+            from .autotune import table_for
+            rows = table_for(order, variant, nbody)
+            if rows:
+                self.set_tuning_table(rows)
+                self.tuning_rows = rows
 
     def _check(self, rc, what):
         if rc != 0:
@@ -214,6 +233,16 @@ class Gravity:
     def set_tuning_ex(self, i_per_thread: int = 0, split_fold: int = 0):
         self._check(hip_lib().hh_set_tuning_ex(self.h, i_per_thread, split_fold), "hh_set_tuning_ex")
 
+    def set_tuning_table(self, rows):
+        """rows: [(nact_hi, kernel, ni, jsplit), ...] ascending in nact_hi;
+        an empty list restores the built-in choice."""
+        n = len(rows)
+        cols = [(C.c_int * max(n, 1))(*[int(r[k]) for r in rows]) for k in range(4)]
+        self._check(hip_lib().hh_set_tuning_table(self.h, n, *cols), "hh_set_tuning_table")
+
+    def load_tuning_file(self, path: str):
+        self._check(hip_lib().hh_load_tuning_file(self.h, path.encode()), "hh_load_tuning_file")
+
     def time_force(self, nact: int, eps2: float, reps: int = 10) -> float:
         ms = C.c_double()
         self._check(hip_lib().hh_time_force(self.h, nact, eps2, reps, C.byref(ms)), "hh_time_force")
@@ -247,7 +276,7 @@ class System:
 
     PHASES = ("force", "pot", "predict", "correct", "sort", "scan", "set_jp", "total")
 
-    def __init__(self, order, P, tsys, eps2, eta, eta_s, dtmax, dtmin=0.0, variant="fp64", device=0):
+    def __init__(self, order, P, tsys, eps2, eta, eta_s, dtmax, dtmin=0.0, variant="fp64", device=0, tuning=True):
         L = host_lib()
         self.order, self.n = order, len(P)
         P = np.ascontiguousarray(P, dtype=np.float64)
@@ -255,6 +284,15 @@ class System:
         if not self.h:
             raise HHError("hhd_create: bad arguments")
         self._check(L.hhd_status(self.h), "hhd_create")
+        if tuning and variant != "exact":
+            from .autotune import table_for
+            rows = table_for(order, variant, self.n)
+            if rows:
+                n = len(rows)
+                cols = [(C.c_int * n)(*[int(r[k]) for r in rows]) for k in range(4)]
+                rc = hip_lib().hh_set_tuning_table(L.hhd_gravity(self.h), n, *cols)
+                if rc != 0:
+                    raise HHError(f"hh_set_tuning_table failed ({rc})")
 
     def _check(self, rc, what):
         if rc < 0:
@@ -374,3 +412,17 @@ class Resident:
     @property
     def steps_launched(self):
         return hip_lib().hhr_steps_launched(self.h)
+
+    def set_plan(self, jpar_max=64, smax=256, task_target=4096, jpar_target=256):
+        """Work split of the persistent force kernel (see hhr_set_plan)."""
+        self._check(hip_lib().hhr_set_plan(self.h, jpar_max, smax, task_target, jpar_target), "hhr_set_plan")
+
+    def set_timing(self, on=True):
+        """Bracket every force launch with HIP events (resets the totals)."""
+        self._check(hip_lib().hhr_set_timing(self.h, int(on)), "hhr_set_timing")
+
+    def force_timing(self):
+        """(summed force-kernel device ms, launches) since set_timing."""
+        ms, n = C.c_double(), C.c_longlong()
+        self._check(hip_lib().hhr_force_timing(self.h, C.byref(ms), C.byref(n)), "hhr_force_timing")
+        return ms.value, n.value

@@ -1,0 +1,163 @@
+"""Autotuner for the force launch geometry.
+
+For one (order, variant, N) it times every candidate geometry of the force
+call (i-parallel tiles with 1 or 2 i per lane and a j-split, or the
+j-parallel kernel with a j-split) at active counts 1, 2, 4, ..., N with HIP
+events on the device (hh_time_force), keeps the fastest per active count and
+turns the winners into a table of rows (nact_hi, kernel, ni, jsplit): a row
+covers the active counts up to the geometric midpoint between its measured
+count and the next one.  The reference fixes its geometry at compile time
+(NJBLOCK/NXREDUCE, SRC/hermite6-titan.h:7-14, and the AVX back end's fixed
+SIMD grouping); here it is measured per device and per N.
+
+Tables live in one JSON file keyed "order/variant/N" (tuning/mi355x.json
+next to this module); Gravity contexts load the row set of the nearest tuned
+N automatically (HERMITE_HIP_TUNING=off disables it, a path selects another
+file).  The C++ drop-in takes the text form written by write_text through
+hh_load_tuning_file.
+
+    python -m hermite_amd.autotune --orders 4 6 8 --variants fp64 split \\
+        --sizes 1024 4096 16384 65536 262144
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+DEFAULT_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning", "mi355x.json")
+INT_MAX = 2**31 - 1
+
+
+def candidates(order, variant, nbody, nact):
+    """Launch geometries (kernel, ni, jsplit) worth timing at this size."""
+    ntiles = (nbody + 255) // 256
+    splits = [0] + [s for s in (1, 2, 4, 8, 16, 32, 64, 128, 256) if s <= ntiles]
+    iblocks = max(1, nact // 256)
+    # more than ~8192 workgroups never pays; skip those splits
+    splits = [s for s in splits if s == 0 or s * iblocks <= 8192]
+    out = [(1, 1, s) for s in splits]
+    if variant != "exact":
+        out += [(1, 2, s) for s in splits if s * max(1, nact // 512) <= 8192]
+        if nact <= 256:
+            out += [(2, 1, s) for s in [0] + [s for s in (4, 8, 16, 32, 64, 128, 256) if s <= ntiles]]
+    return out
+
+
+def _state(order, nbody, seed=7):
+    import hermite_amd as H
+    m, x, v = H.plummer(nbody, seed)
+    P = H.make_particles(order, m, x, v)
+    rng = np.random.default_rng(seed)
+    # plausible stored derivatives so the predictor does real work
+    P[:, 10:] = 1e-3 * rng.standard_normal(P[:, 10:].shape)
+    return P
+
+
+def tune(order, variant, nbody, reps=None, device=0, log=None):
+    """Measure the candidates at active counts 1, 2, 4, ..., nbody and return
+    (rows, measurements)."""
+    import hermite_amd as H
+    eps2 = (4.0 / nbody) ** 2
+    g = H.Gravity(order, nbody, variant, device=device, tuning=False)
+    try:
+        g.set_jp_all(_state(order, nbody))
+        g.predict_all(1e-3)
+        nacts = sorted(set([1 << k for k in range(int(math.log2(nbody)) + 1)] + [nbody]))
+        best, meas = [], {}
+        for nact in nacts:
+            work = nact * nbody
+            r = reps or max(3, min(50, int(2e9 / max(work, 1))))
+            res = []
+            for (k, ni, s) in candidates(order, variant, nbody, nact):
+                g.set_tuning(k, s)
+                g.set_tuning_ex(ni, 0)
+                g.time_force(nact, eps2, 1)  # warm (allocates partials)
+                ms = g.time_force(nact, eps2, r)
+                res.append((ms, k, ni, s))
+            res.sort()
+            meas[nact] = [(round(ms * 1e3, 2), k, ni, s) for ms, k, ni, s in res]
+            ms, k, ni, s = res[0]
+            best.append((nact, k, ni, s, ms))
+            if log:
+                print(f"order={order} {variant} N={nbody} nact={nact}: best k{k} ni{ni} s{s} {ms * 1e3:.1f} us "
+                      f"({work / (ms * 1e-3):.3e} int/s)", file=log, flush=True)
+    finally:
+        g.close()
+    rows = []
+    for q, (nact, k, ni, s, _) in enumerate(best):
+        hi = INT_MAX if q + 1 == len(best) else int(math.sqrt(nact * best[q + 1][0]))
+        if rows and rows[-1][1:] == (k, ni, s):
+            rows[-1] = (hi, k, ni, s)
+        else:
+            rows.append((hi, k, ni, s))
+    return rows, meas
+
+
+def key(order, variant, nbody):
+    return f"{order}/{variant}/{nbody}"
+
+
+def load(path=None):
+    path = path or os.environ.get("HERMITE_HIP_TUNING") or DEFAULT_PATH
+    if path == "off" or not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f)
+
+
+def table_for(order, variant, nbody, path=None):
+    """Rows tuned for the nearest N (in log2) of this order and variant, or
+    None when the file holds none within a factor of 4."""
+    tables = load(path)
+    best, dist = None, 2.0 + 1e-9
+    for k, rows in tables.items():
+        o, v, n = k.split("/")
+        if int(o) != order or v != variant:
+            continue
+        d = abs(math.log2(int(n)) - math.log2(nbody))
+        if d <= dist:
+            best, dist = rows, d
+    return None if best is None else [tuple(r) for r in best]
+
+
+def write_text(rows, path):
+    """Text form read by hh_load_tuning_file: "nact_hi kernel ni jsplit"."""
+    with open(path, "w") as f:
+        f.write("# nact_hi kernel ni jsplit\n")
+        for r in rows:
+            f.write("%d %d %d %d\n" % tuple(r))
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--orders", type=int, nargs="+", default=[4, 6, 8])
+    ap.add_argument("--variants", nargs="+", default=["fp64", "split"])
+    ap.add_argument("--sizes", type=int, nargs="+", default=[1024, 4096, 16384, 65536])
+    ap.add_argument("--out", default=DEFAULT_PATH)
+    ap.add_argument("--log", default=None, help="also write every measurement here (JSON)")
+    ap.add_argument("--device", type=int, default=0)
+    a = ap.parse_args(argv)
+    tables = load(a.out) if os.path.exists(a.out) else {}
+    allmeas = {}
+    t0 = time.time()
+    for order in a.orders:
+        for variant in a.variants:
+            for n in a.sizes:
+                rows, meas = tune(order, variant, n, device=a.device, log=sys.stdout)
+                tables[key(order, variant, n)] = [list(r) for r in rows]
+                allmeas[key(order, variant, n)] = meas
+                print(f"{key(order, variant, n)}: {rows}  ({time.time() - t0:.0f} s)", flush=True)
+    os.makedirs(os.path.dirname(a.out), exist_ok=True)
+    with open(a.out, "w") as f:
+        json.dump(tables, f, indent=1, sort_keys=True)
+    if a.log:
+        with open(a.log, "w") as f:
+            json.dump({k: {str(n): v for n, v in m.items()} for k, m in allmeas.items()}, f)
+
+
+if __name__ == "__main__":
+    main()

@@ -521,6 +521,7 @@ void hhd_destroy(hhd_sys *s)
 }
 
 int hhd_status(const hhd_sys *s) { return s ? s->rc : HH_E_ARG; }
+hh_ctx *hhd_gravity(hhd_sys *s) { return s ? s->g : nullptr; }
 const char *hhd_error(const hhd_sys *s) { return s ? s->err.c_str() : "null driver"; }
 
 int hhd_init_step(hhd_sys *s)

@@ -15,9 +15,20 @@
 #include <atomic>
 #include <cstdio>
 #include <cstring>
+#include <cstddef>
 #include <string>
+#include <vector>
 
 using namespace hh;
+
+// One row of a force-launch tuning table: active counts up to nact_hi use
+// this kernel (1 i-parallel, 2 j-parallel), i per lane and j-slice count.
+struct TuneEntry {
+    int nact_hi, kernel, ni, jsplit;
+};
+struct Geo {
+    int kernel, ni, jsplit;
+};
 
 struct hh_ctx {
     int order = 0, nbody = 0, variant = 0, device = 0, npad = 0;
@@ -35,6 +46,7 @@ struct hh_ctx {
     size_t part_cap = 0;        // doubles
     std::atomic<int> dirty_hi{0};
     int tune_kernel = 0, tune_split = 0, tune_ni = 0, tune_fold = 0;
+    std::vector<TuneEntry> table;  // ascending nact_hi; empty = built-in choice
     int last_kernel = 0, last_split = 0, last_wgs = 0;
     std::string err;
 };
@@ -160,7 +172,7 @@ static hipError_t launch_reduce(hh_ctx *c, const double *part, int nslice, int n
 
 // i-parallel launch of tile_kernel over src slots [0, ni) as i-particles.
 template <int ORDER, int MODE, bool POT, int NI, int FOLD>
-static int launch_tile_ni(hh_ctx *c, const double *src, int ni, double eps2, double *fout, double *pout)
+static int launch_tile_ni(hh_ctx *c, const Geo &g, const double *src, int ni, double eps2, double *fout, double *pout)
 {
     constexpr int NC = n_acc(ORDER, POT);
     const int bi = NT * NI;
@@ -168,7 +180,7 @@ static int launch_tile_ni(hh_ctx *c, const double *src, int ni, double eps2, dou
     const int ntiles = (c->nbody + TJ - 1) / TJ;
     int nslice;
     if (MODE == EXACT) nslice = 1;
-    else if (c->tune_split > 0) nslice = std::min(c->tune_split, ntiles);
+    else if (g.jsplit > 0) nslice = std::min(g.jsplit, ntiles);
     else nslice = std::max(1, std::min(ntiles, (TARGET_WG + iblocks - 1) / iblocks));
     const int tps = (ntiles + nslice - 1) / nslice;
     nslice = (ntiles + tps - 1) / tps;
@@ -206,28 +218,28 @@ static int launch_tile_ni(hh_ctx *c, const double *src, int ni, double eps2, dou
 static constexpr int DEFAULT_FOLD = 8;
 
 template <int ORDER, int MODE, bool POT>
-static int launch_tile(hh_ctx *c, const double *src, int ni, double eps2, double *fout, double *pout)
+static int launch_tile(hh_ctx *c, const Geo &g, const double *src, int ni, double eps2, double *fout, double *pout)
 {
     if constexpr (MODE == EXACT) {
-        return launch_tile_ni<ORDER, MODE, POT, 1, 1>(c, src, ni, eps2, fout, pout);
+        return launch_tile_ni<ORDER, MODE, POT, 1, 1>(c, g, src, ni, eps2, fout, pout);
     } else if constexpr (MODE == FP64) {
-        if (c->tune_ni == 2) return launch_tile_ni<ORDER, MODE, POT, 2, 1>(c, src, ni, eps2, fout, pout);
-        return launch_tile_ni<ORDER, MODE, POT, 1, 1>(c, src, ni, eps2, fout, pout);
+        if (g.ni == 2) return launch_tile_ni<ORDER, MODE, POT, 2, 1>(c, g, src, ni, eps2, fout, pout);
+        return launch_tile_ni<ORDER, MODE, POT, 1, 1>(c, g, src, ni, eps2, fout, pout);
     } else {
         const int fold = c->tune_fold ? c->tune_fold : DEFAULT_FOLD;
-        if (c->tune_ni == 2) {
+        if (g.ni == 2) {
             switch (fold) {
-            case 4: return launch_tile_ni<ORDER, MODE, POT, 2, 4>(c, src, ni, eps2, fout, pout);
-            case 16: return launch_tile_ni<ORDER, MODE, POT, 2, 16>(c, src, ni, eps2, fout, pout);
-            case 32: return launch_tile_ni<ORDER, MODE, POT, 2, 32>(c, src, ni, eps2, fout, pout);
-            default: return launch_tile_ni<ORDER, MODE, POT, 2, 8>(c, src, ni, eps2, fout, pout);
+            case 4: return launch_tile_ni<ORDER, MODE, POT, 2, 4>(c, g, src, ni, eps2, fout, pout);
+            case 16: return launch_tile_ni<ORDER, MODE, POT, 2, 16>(c, g, src, ni, eps2, fout, pout);
+            case 32: return launch_tile_ni<ORDER, MODE, POT, 2, 32>(c, g, src, ni, eps2, fout, pout);
+            default: return launch_tile_ni<ORDER, MODE, POT, 2, 8>(c, g, src, ni, eps2, fout, pout);
             }
         }
         switch (fold) {
-        case 4: return launch_tile_ni<ORDER, MODE, POT, 1, 4>(c, src, ni, eps2, fout, pout);
-        case 16: return launch_tile_ni<ORDER, MODE, POT, 1, 16>(c, src, ni, eps2, fout, pout);
-        case 32: return launch_tile_ni<ORDER, MODE, POT, 1, 32>(c, src, ni, eps2, fout, pout);
-        default: return launch_tile_ni<ORDER, MODE, POT, 1, 8>(c, src, ni, eps2, fout, pout);
+        case 4: return launch_tile_ni<ORDER, MODE, POT, 1, 4>(c, g, src, ni, eps2, fout, pout);
+        case 16: return launch_tile_ni<ORDER, MODE, POT, 1, 16>(c, g, src, ni, eps2, fout, pout);
+        case 32: return launch_tile_ni<ORDER, MODE, POT, 1, 32>(c, g, src, ni, eps2, fout, pout);
+        default: return launch_tile_ni<ORDER, MODE, POT, 1, 8>(c, g, src, ni, eps2, fout, pout);
         }
     }
 }
@@ -238,12 +250,12 @@ static constexpr int TARGET_WG_JPAR = 1024;
 
 // j-parallel launch (jpar_kernel) over [0, nact) as i-particles.
 template <int ORDER, int MODE, int GI, bool POT>
-static int launch_jpar(hh_ctx *c, int nact, double eps2)
+static int launch_jpar(hh_ctx *c, const Geo &g, int nact, double eps2)
 {
     constexpr int V = GI * n_acc(ORDER, POT);
     const int ngroups = (nact + GI - 1) / GI;
     const int ntj = c->npad / NT;
-    int nslice = c->tune_split > 0 ? c->tune_split : (TARGET_WG_JPAR + ngroups - 1) / ngroups;
+    int nslice = g.jsplit > 0 ? g.jsplit : (TARGET_WG_JPAR + ngroups - 1) / ngroups;
     nslice = std::max(1, std::min(nslice, ntj));
     const int jchunk = (ntj + nslice - 1) / nslice * NT;
     nslice = (c->npad + jchunk - 1) / jchunk;
@@ -276,22 +288,37 @@ static int launch_jpar(hh_ctx *c, int nact, double eps2)
 }
 
 template <int ORDER, int MODE, bool POT>
-static int launch_jpar_gi(hh_ctx *c, int nact, double eps2)
+static int launch_jpar_gi(hh_ctx *c, const Geo &g, int nact, double eps2)
 {
-    if (nact == 1) return launch_jpar<ORDER, MODE, 1, POT>(c, nact, eps2);
-    if (nact == 2) return launch_jpar<ORDER, MODE, 2, POT>(c, nact, eps2);
-    return launch_jpar<ORDER, MODE, 4, POT>(c, nact, eps2);
+    if (nact == 1) return launch_jpar<ORDER, MODE, 1, POT>(c, g, nact, eps2);
+    if (nact == 2) return launch_jpar<ORDER, MODE, 2, POT>(c, g, nact, eps2);
+    return launch_jpar<ORDER, MODE, 4, POT>(c, g, nact, eps2);
+}
+
+// Launch geometry for an active count: the tuning table row covering nact
+// (or the built-in choice), then any explicit hh_set_tuning overrides.
+static Geo choose_geometry(const hh_ctx *c, int nact)
+{
+    Geo g{nact <= JPAR_MAX ? 2 : 1, 1, 0};
+    for (const TuneEntry &t : c->table)
+        if (nact <= t.nact_hi) {
+            g = Geo{t.kernel, t.ni, t.jsplit};
+            break;
+        }
+    if (c->tune_kernel) g.kernel = c->tune_kernel;
+    if (c->tune_ni) g.ni = c->tune_ni;
+    if (c->tune_split) g.jsplit = c->tune_split;
+    return g;
 }
 
 template <int ORDER, int MODE, bool POT>
 static int launch_force_mode(hh_ctx *c, int nact, double eps2)
 {
-    int kernel = c->tune_kernel;
-    if (kernel == 0) kernel = nact <= JPAR_MAX ? 2 : 1;
+    const Geo g = choose_geometry(c, nact);
     if constexpr (MODE != EXACT) {
-        if (kernel == 2) return launch_jpar_gi<ORDER, MODE, POT>(c, nact, eps2);
+        if (g.kernel == 2) return launch_jpar_gi<ORDER, MODE, POT>(c, g, nact, eps2);
     }
-    return launch_tile<ORDER, MODE, POT>(c, c->d_jpr, nact, eps2, c->d_force, c->d_pot);
+    return launch_tile<ORDER, MODE, POT>(c, g, c->d_jpr, nact, eps2, c->d_force, c->d_pot);
 }
 
 template <int ORDER>
@@ -318,10 +345,11 @@ static int launch_force(hh_ctx *c, int nact, double eps2, bool pot)
 
 static int launch_potential(hh_ctx *c, double eps2)
 {
+    const Geo g{1, 1, 0};
     switch (c->variant) {
-    case HH_FP64: return launch_tile<0, FP64, false>(c, c->d_jst, c->nbody, eps2, c->d_pot, nullptr);
-    case HH_SPLIT: return launch_tile<0, SPLIT, false>(c, c->d_jst, c->nbody, eps2, c->d_pot, nullptr);
-    default: return launch_tile<0, EXACT, false>(c, c->d_jst, c->nbody, eps2, c->d_pot, nullptr);
+    case HH_FP64: return launch_tile<0, FP64, false>(c, g, c->d_jst, c->nbody, eps2, c->d_pot, nullptr);
+    case HH_SPLIT: return launch_tile<0, SPLIT, false>(c, g, c->d_jst, c->nbody, eps2, c->d_pot, nullptr);
+    default: return launch_tile<0, EXACT, false>(c, g, c->d_jst, c->nbody, eps2, c->d_pot, nullptr);
     }
 }
 
@@ -473,6 +501,43 @@ int hh_set_tuning(hh_ctx *c, int kernel, int jsplit)
     return HH_OK;
 }
 
+int hh_set_tuning_table(hh_ctx *c, int n, const int *nact_hi, const int *kernel, const int *ni, const int *jsplit)
+{
+    if (!c || n < 0 || (n > 0 && (!nact_hi || !kernel || !ni || !jsplit))) return HH_E_ARG;
+    std::vector<TuneEntry> t;
+    for (int k = 0; k < n; k++) {
+        if (kernel[k] < 1 || kernel[k] > 2 || ni[k] < 1 || ni[k] > 2 || jsplit[k] < 0) return HH_E_ARG;
+        if (k > 0 && nact_hi[k] <= nact_hi[k - 1]) return HH_E_ARG;
+        t.push_back(TuneEntry{nact_hi[k], kernel[k], ni[k], jsplit[k]});
+    }
+    c->table.swap(t);
+    return HH_OK;
+}
+
+// Text table: one row per line, "nact_hi kernel ni jsplit", ascending.
+int hh_load_tuning_file(hh_ctx *c, const char *path)
+{
+    if (!c || !path) return HH_E_ARG;
+    FILE *fp = std::fopen(path, "r");
+    if (!fp) return fail(c, HH_E_ARG, std::string("cannot open tuning table ") + path);
+    std::vector<int> a, b, d, e;
+    char line[256];
+    while (std::fgets(line, sizeof line, fp)) {
+        int x0, x1, x2, x3;
+        if (line[0] == '#') continue;
+        if (std::sscanf(line, "%d %d %d %d", &x0, &x1, &x2, &x3) == 4) {
+            a.push_back(x0);
+            b.push_back(x1);
+            d.push_back(x2);
+            e.push_back(x3);
+        }
+    }
+    std::fclose(fp);
+    const int r = hh_set_tuning_table(c, (int)a.size(), a.data(), b.data(), d.data(), e.data());
+    if (r) return fail(c, r, std::string("malformed tuning table ") + path);
+    return HH_OK;
+}
+
 int hh_set_tuning_ex(hh_ctx *c, int i_per_thread, int split_fold)
 {
     if (!c || i_per_thread < 0 || i_per_thread > 2) return HH_E_ARG;
@@ -576,6 +641,12 @@ struct hhr_ctx {
     int graph_steps = 0;
     double last_run_ms = 0.0;
     long long steps_launched = 0;
+    // Optional per-launch timing of the force kernel: an event pair around
+    // every force launch of a batch (event record nodes inside a graph).
+    bool timing = false, capturing = false;
+    std::vector<hipEvent_t> kev;
+    double force_ms = 0.0;
+    long long force_launches = 0;
     std::string err;
 };
 
@@ -611,32 +682,50 @@ static RArgs r_args(hhr_ctx *c)
     return a;
 }
 
+static hipError_t r_mark(hhr_ctx *c, int k)
+{
+    if (!c->timing) return hipSuccess;
+    return c->capturing ? hipEventRecordWithFlags(c->kev[k], c->stream, hipEventRecordExternal)
+                        : hipEventRecord(c->kev[k], c->stream);
+}
+
+// One block step: six launches; s is the step's index within its batch.
 template <int ORDER, int MODE>
-static hipError_t r_launch_step(hhr_ctx *c)
+static hipError_t r_launch_step(hhr_ctx *c, int s)
 {
     const RArgs a = r_args(c);
     hipLaunchKernelGGL(r_min_kernel, dim3(c->ntiles), dim3(NT), 0, c->stream, a);
     hipLaunchKernelGGL(r_compact_kernel, dim3(c->ntiles), dim3(NT), 0, c->stream, a);
     hipLaunchKernelGGL((r_plan_kernel<ORDER>), dim3(1), dim3(NT), 0, c->stream, a);
     hipLaunchKernelGGL((r_predict_kernel<ORDER, MODE == EXACT>), dim3(c->ntiles), dim3(NT), 0, c->stream, a);
+    hipError_t e = r_mark(c, 2 * s);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL((r_force_kernel<ORDER, MODE>), dim3(c->force_grid), dim3(NT), 0, c->stream, a);
+    e = r_mark(c, 2 * s + 1);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL((r_reduce_correct_kernel<ORDER>), dim3(c->force_grid), dim3(NT), 0, c->stream, a);
     return hipGetLastError();
 }
 
-static hipError_t r_step(hhr_ctx *c)
+static hipError_t r_step(hhr_ctx *c, int s)
 {
     switch (c->order * 4 + c->variant) {
-    case 16 + HH_FP64: return r_launch_step<4, FP64>(c);
-    case 16 + HH_SPLIT: return r_launch_step<4, SPLIT>(c);
-    case 16 + HH_EXACT: return r_launch_step<4, EXACT>(c);
-    case 24 + HH_FP64: return r_launch_step<6, FP64>(c);
-    case 24 + HH_SPLIT: return r_launch_step<6, SPLIT>(c);
-    case 24 + HH_EXACT: return r_launch_step<6, EXACT>(c);
-    case 32 + HH_FP64: return r_launch_step<8, FP64>(c);
-    case 32 + HH_SPLIT: return r_launch_step<8, SPLIT>(c);
-    default: return r_launch_step<8, EXACT>(c);
+    case 16 + HH_FP64: return r_launch_step<4, FP64>(c, s);
+    case 16 + HH_SPLIT: return r_launch_step<4, SPLIT>(c, s);
+    case 16 + HH_EXACT: return r_launch_step<4, EXACT>(c, s);
+    case 24 + HH_FP64: return r_launch_step<6, FP64>(c, s);
+    case 24 + HH_SPLIT: return r_launch_step<6, SPLIT>(c, s);
+    case 24 + HH_EXACT: return r_launch_step<6, EXACT>(c, s);
+    case 32 + HH_FP64: return r_launch_step<8, FP64>(c, s);
+    case 32 + HH_SPLIT: return r_launch_step<8, SPLIT>(c, s);
+    default: return r_launch_step<8, EXACT>(c, s);
     }
+}
+
+static void r_free_events(hhr_ctx *c)
+{
+    for (hipEvent_t e : c->kev) (void)hipEventDestroy(e);
+    c->kev.clear();
 }
 
 static void r_free(hhr_ctx *c)
@@ -655,6 +744,7 @@ static void r_free(hhr_ctx *c)
     (void)hipFree(c->glob);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    r_free_events(c);
     if (c->stream) (void)hipStreamDestroy(c->stream);
 }
 
@@ -762,6 +852,7 @@ int hhr_create(int order, int variant, int device, int nclu, const int *ns, cons
     std::memset(&g, 0, sizeof(g));
     g.nclu = nclu;
     g.exact = variant == HH_EXACT;
+    g.plan = RPlan{};
 
     auto bail = [&](const char *what, hipError_t e) {
         g_hhr_err = std::string(what) + ": " + hipGetErrorString(e);
@@ -770,9 +861,11 @@ int hhr_create(int order, int variant, int device, int nclu, const int *ns, cons
         return HH_E_HIP;
     };
     hipError_t e;
-    // partials: one slice of every active particle (nslice 1) or, when
-    // sliced, fewer than two budgets of tasks per cluster (plan kernel)
-    c->part_cap = (size_t)F * (tot + (size_t)NT * nclu) + (size_t)F * NT * 2 * (R_TASK_TARGET + nclu);
+    // Partials of one block step: i-parallel clusters hold nslice * ngrp <=
+    // budget + ngrp tasks of NT i each, budgets summing to at most
+    // task_target + nclu, and ngrp summing to at most (tot + NT nclu) / NT;
+    // j-parallel clusters need less.  Sized for the largest task target.
+    c->part_cap = (size_t)F * (tot + (size_t)NT * nclu) + (size_t)F * NT * 2 * (R_TASK_MAX + nclu);
     if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) return bail("stream", e);
     if ((e = hipEventCreate(&c->ev0)) != hipSuccess) return bail("event", e);
     if ((e = hipEventCreate(&c->ev1)) != hipSuccess) return bail("event", e);
@@ -818,14 +911,25 @@ int hhr_run(hhr_ctx *c, double tend, int use_graph, int batch_steps)
     RCHK(c, hipMemcpy(hc.data(), c->clu, sizeof(RClu) * c->nclu, hipMemcpyDeviceToHost));
     for (auto &h : hc) h.tend = tend;
     RCHK(c, hipMemcpy(c->clu, hc.data(), sizeof(RClu) * c->nclu, hipMemcpyHostToDevice));
+    if (c->timing && (int)c->kev.size() != 2 * batch_steps) {
+        if (c->gexec) RCHK(c, hipGraphExecDestroy(c->gexec));
+        if (c->graph) RCHK(c, hipGraphDestroy(c->graph));
+        c->gexec = nullptr;
+        c->graph = nullptr;
+        r_free_events(c);
+        c->kev.assign(2 * batch_steps, nullptr);
+        for (auto &e : c->kev) RCHK(c, hipEventCreate(&e));
+    }
     if (use_graph && (!c->gexec || c->graph_steps != batch_steps)) {
         if (c->gexec) RCHK(c, hipGraphExecDestroy(c->gexec));
         if (c->graph) RCHK(c, hipGraphDestroy(c->graph));
         c->gexec = nullptr;
         c->graph = nullptr;
         RCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        c->capturing = true;
         hipError_t e = hipSuccess;
-        for (int s = 0; s < batch_steps && e == hipSuccess; s++) e = r_step(c);
+        for (int s = 0; s < batch_steps && e == hipSuccess; s++) e = r_step(c, s);
+        c->capturing = false;
         hipGraph_t gph = nullptr;
         const hipError_t e2 = hipStreamEndCapture(c->stream, &gph);
         if (e != hipSuccess) return rfail(c, HH_E_HIP, std::string("step capture: ") + hipGetErrorString(e));
@@ -841,13 +945,21 @@ int hhr_run(hhr_ctx *c, double tend, int use_graph, int batch_steps)
             RCHK(c, hipGraphLaunch(c->gexec, c->stream));
         } else {
             for (int s = 0; s < batch_steps; s++) {
-                const hipError_t e = r_step(c);
+                const hipError_t e = r_step(c, s);
                 if (e != hipSuccess) return rfail(c, HH_E_HIP, std::string("step launch: ") + hipGetErrorString(e));
             }
         }
         c->steps_launched += batch_steps;
         RCHK(c, hipMemcpyAsync(&g, c->glob, sizeof(RGlob), hipMemcpyDeviceToHost, c->stream));
         RCHK(c, hipStreamSynchronize(c->stream));
+        if (c->timing) {
+            for (int s = 0; s < batch_steps; s++) {
+                float kms = 0.f;
+                RCHK(c, hipEventElapsedTime(&kms, c->kev[2 * s], c->kev[2 * s + 1]));
+                c->force_ms += kms;
+            }
+            c->force_launches += batch_steps;
+        }
         if (g.error == 1) return rfail(c, HH_E_STATE, "a cluster's block time stopped advancing (step-size collapse)");
         if (g.error == 2) return rfail(c, HH_E_STATE, "a particle state became non-finite");
         if (g.nactive == 0) break;
@@ -874,6 +986,49 @@ int hhr_stats(hhr_ctx *c, int k, long long *num_step, long long *num_bstep, doub
 }
 
 double hhr_last_run_ms(const hhr_ctx *c) { return c ? c->last_run_ms : -1.0; }
+
+int hhr_set_plan(hhr_ctx *c, int jpar_max, int smax, int task_target, int jpar_target)
+{
+    if (!c) return HH_E_ARG;
+    if (jpar_max < 0 || smax < 1 || task_target < 1 || task_target > R_TASK_MAX || jpar_target < 1 ||
+        jpar_target > R_TASK_MAX)
+        return rfail(c, HH_E_ARG, "plan parameters out of range");
+    RCHK(c, hipSetDevice(c->device));
+    RCHK(c, hipStreamSynchronize(c->stream));
+    RPlan p;
+    p.jpar_max = jpar_max;
+    p.smax = smax;
+    p.task_target = task_target;
+    p.jpar_target = jpar_target;
+    RCHK(c, hipMemcpy((char *)c->glob + offsetof(RGlob, plan), &p, sizeof(RPlan), hipMemcpyHostToDevice));
+    return HH_OK;
+}
+
+int hhr_set_timing(hhr_ctx *c, int on)
+{
+    if (!c) return HH_E_ARG;
+    if (c->timing != (on != 0)) {
+        RCHK(c, hipSetDevice(c->device));
+        RCHK(c, hipStreamSynchronize(c->stream));
+        if (c->gexec) RCHK(c, hipGraphExecDestroy(c->gexec));
+        if (c->graph) RCHK(c, hipGraphDestroy(c->graph));
+        c->gexec = nullptr;
+        c->graph = nullptr;
+        r_free_events(c);
+    }
+    c->timing = on != 0;
+    c->force_ms = 0.0;
+    c->force_launches = 0;
+    return HH_OK;
+}
+
+int hhr_force_timing(const hhr_ctx *c, double *ms_total, long long *launches)
+{
+    if (!c || !ms_total || !launches) return HH_E_ARG;
+    *ms_total = c->force_ms;
+    *launches = c->force_launches;
+    return HH_OK;
+}
 long long hhr_steps_launched(const hhr_ctx *c) { return c ? c->steps_launched : -1; }
 
 // Reference Particle records of cluster k.
@@ -933,10 +1088,11 @@ int hhr_energy(hhr_ctx *c, int k, double *energy)
     tmp.d_part = c->part;
     tmp.part_cap = c->part_cap;
     int r;
+    const Geo g{1, 1, 0};
     switch (c->variant) {
-    case HH_FP64: r = launch_tile<0, FP64, false>(&tmp, c->rp + o, nk, c->eps2[k], c->pot, nullptr); break;
-    case HH_SPLIT: r = launch_tile<0, SPLIT, false>(&tmp, c->rp + o, nk, c->eps2[k], c->pot, nullptr); break;
-    default: r = launch_tile<0, EXACT, false>(&tmp, c->rp + o, nk, c->eps2[k], c->pot, nullptr); break;
+    case HH_FP64: r = launch_tile<0, FP64, false>(&tmp, g, c->rp + o, nk, c->eps2[k], c->pot, nullptr); break;
+    case HH_SPLIT: r = launch_tile<0, SPLIT, false>(&tmp, g, c->rp + o, nk, c->eps2[k], c->pot, nullptr); break;
+    default: r = launch_tile<0, EXACT, false>(&tmp, g, c->rp + o, nk, c->eps2[k], c->pot, nullptr); break;
     }
     tmp.stream = nullptr;
     tmp.d_part = nullptr;

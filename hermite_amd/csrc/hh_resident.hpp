@@ -30,9 +30,16 @@ constexpr int R_DT = 26, R_ID = 27, R_D6A = 28, R_D7A = 31;
 __host__ __device__ constexpr int r_words(int o) { return o == 8 ? 34 : 28; }
 
 constexpr int R_GI = 4;          // i per group of the j-parallel task
-constexpr int R_JPAR_MAX = 64;   // active counts up to this use j-parallel tasks
-constexpr int R_SMAX = 64;       // most j-slices per i-group
-constexpr int R_TASK_TARGET = 2048;
+constexpr int R_TASK_MAX = 8192; // largest task target a plan may ask for
+
+// Plan parameters (hhr_set_plan): active counts up to jpar_max use
+// j-parallel groups; otherwise i-parallel groups of NT with at most smax
+// j-slices.  A block step aims at task_target i-parallel tasks
+// (jpar_target j-parallel ones) over all clusters, shared in proportion to
+// each cluster's nact * n.
+struct RPlan {
+    int jpar_max = 64, smax = 256, task_target = 4096, jpar_target = 256;
+};
 
 struct RClu {
     double tsys, tend, dtmax, dtmin, eps2, etapow;
@@ -52,6 +59,7 @@ struct RGlob {
     int error;     // a cluster's block time stopped advancing
     int total_active;
     int pad;
+    RPlan plan;
 };
 
 struct RArgs {
@@ -322,14 +330,17 @@ __global__ __launch_bounds__(NT) void r_plan_kernel(RArgs a)
         c.num_step += c.nact;
         c.num_bstep += 1;
         c.tsys = tmin;
-        const int budget = max(1, (int)ceil(R_TASK_TARGET * ((double)c.nact * c.n) / total));
-        if (a.glob->exact || c.nact > R_JPAR_MAX) {
+        const RPlan &pl = a.glob->plan;
+        const double share = ((double)c.nact * c.n) / total;
+        if (a.glob->exact || c.nact > pl.jpar_max) {
+            const int budget = max(1, (int)ceil(pl.task_target * share));
             c.mode = 1;
             c.ngrp = (c.nact + NT - 1) / NT;
-            int ns = a.glob->exact ? 1 : min(min(c.ntiles, R_SMAX), max(1, (budget + c.ngrp - 1) / c.ngrp));
+            int ns = a.glob->exact ? 1 : min(min(c.ntiles, pl.smax), max(1, (budget + c.ngrp - 1) / c.ngrp));
             c.tps = (c.ntiles + ns - 1) / ns;
             c.nslice = (c.ntiles + c.tps - 1) / c.tps;
         } else {
+            const int budget = max(1, (int)ceil(pl.jpar_target * share));
             c.mode = 2;
             c.ngrp = (c.nact + R_GI - 1) / R_GI;
             int ns = min(c.ntiles, max(1, (budget + c.ngrp - 1) / c.ngrp));

@@ -122,6 +122,15 @@ int hh_set_tuning(hh_ctx *ctx, int kernel, int jsplit);
  * 0 = default). */
 int hh_set_tuning_ex(hh_ctx *ctx, int i_per_thread, int split_fold);
 
+/* Tuning table (written by hermite_amd/autotune.py): row k applies to active
+ * counts up to nact_hi[k] (ascending): kernel (1 i-parallel, 2 j-parallel),
+ * i per lane (1 or 2) and j-slices (0 = automatic).  n = 0 clears it.
+ * hh_set_tuning overrides take precedence over the table. */
+int hh_set_tuning_table(hh_ctx *ctx, int n, const int *nact_hi, const int *kernel, const int *ni,
+                        const int *jsplit);
+/* Same from a text file of "nact_hi kernel ni jsplit" lines ('#' comments). */
+int hh_load_tuning_file(hh_ctx *ctx, const char *path);
+
 /* Runs the force on [0, nact) reps times on the device without copying the
  * result back, and reports the mean device time per call in milliseconds,
  * measured with HIP events on the context's stream. */
@@ -139,6 +148,59 @@ int hh_synchronize(hh_ctx *ctx);
 
 /* Geometry the last force call used: kernel id, j-slices, workgroups. */
 int hh_last_geometry(const hh_ctx *ctx, int *kernel, int *jsplit, int *workgroups);
+
+/* ---- GPU-resident block-step driver (hhr_*) ----
+ * The whole NbodySystem loop of SRC/nbodysystem.h on the device for one
+ * cluster or an ensemble of independent clusters: block selection by minimum
+ * tlast+dt (integrate_one_block, SRC/nbodysystem.h:693-766, without the
+ * particle permutation: actives are taken in index order), predict, force,
+ * corrector and quantised timestep (SRC/hermite{4,6,8}.h), steps launched in
+ * batches captured into a hipGraph.  The reference runs one system per
+ * process; clusters here share every launch. */
+typedef struct hhr_ctx hhr_ctx;
+
+/* recs: the clusters' reference Particle records (hh_particle_words(order)
+ * doubles each) concatenated, after init_step (SRC/nbodysystem.h:565-631:
+ * derivatives and dt set).  ns[k]: cluster sizes.  params: 5 doubles per
+ * cluster {eps2, eta, dt_max, dt_min, tsys}. */
+int hhr_create(int order, int variant, int device, int nclu, const int *ns, const double *recs,
+               const double *params, hhr_ctx **out);
+int hhr_destroy(hhr_ctx *ctx);
+
+/* Integrate every cluster to tend, a multiple of its dt_max (the loop of
+ * integrate_one_dtmax, SRC/nbodysystem.h:904-944, repeated).  batch_steps
+ * block steps per launch batch (captured once as a hipGraph when use_graph).
+ * Returns HH_E_STATE when a cluster's step collapses (the next block time
+ * does not advance) or its state turns non-finite. */
+int hhr_run(hhr_ctx *ctx, double tend, int use_graph, int batch_steps);
+
+/* Counters of cluster k: particle steps, block steps, system time and the
+ * number of steps clamped to dt_min. */
+int hhr_stats(hhr_ctx *ctx, int k, long long *num_step, long long *num_bstep, double *tsys,
+              long long *dtmin_hits);
+/* Wall milliseconds of the last hhr_run and block steps it launched. */
+double hhr_last_run_ms(const hhr_ctx *ctx);
+long long hhr_steps_launched(const hhr_ctx *ctx);
+/* Work split of the persistent force kernel (no reference counterpart):
+ * active counts up to jpar_max use j-parallel groups of 4 i; larger ones
+ * i-parallel groups of 256 with at most smax j-slices; a block step aims at
+ * task_target i-parallel (jpar_target j-parallel) tasks over all clusters
+  * (both at most 8192).  Defaults 64, 256, 4096, 256 (measured best for a
+ * 6th-order N = 65536 Plummer sphere). */
+int hhr_set_plan(hhr_ctx *ctx, int jpar_max, int smax, int task_target, int jpar_target);
+
+/* Force-kernel timing (no reference counterpart): when on, every force
+ * launch of hhr_run is bracketed by HIP events on the context's stream
+ * (event record nodes inside the graph) and hhr_force_timing reports the
+ * summed device milliseconds and the number of launches since timing was
+ * switched on. */
+int hhr_set_timing(hhr_ctx *ctx, int on);
+int hhr_force_timing(const hhr_ctx *ctx, double *ms_total, long long *launches);
+/* Reference Particle records of cluster k (ns[k] * hh_particle_words). */
+int hhr_get_particles(hhr_ctx *ctx, int k, double *recs);
+/* calc_energy of cluster k (SRC/nbodysystem.h:633-650). */
+int hhr_energy(hhr_ctx *ctx, int k, double *energy);
+const char *hhr_last_error(const hhr_ctx *ctx);
 
 #ifdef __cplusplus
 }

@@ -20,6 +20,9 @@
  * Variant (HH_FP64 / HH_SPLIT / HH_EXACT): compile-time default
  * HERMITE_HIP_VARIANT, overridden at run time by the environment variable
  * HERMITE_HIP_VARIANT=fp64|split|exact.  Device: HERMITE_HIP_DEVICE (0).
+ * Launch geometry: HERMITE_HIP_TUNING_FILE names a tuning table in the text
+ * form of hh_load_tuning_file (written by hermite_amd/autotune.py); without
+ * it the library's built-in choice is used.
  */
 #ifndef HERMITE_HIP_GRAVITY_H
 #define HERMITE_HIP_GRAVITY_H
@@ -77,6 +80,8 @@ struct Gravity {
             std::fprintf(stderr, "hh_create failed (%d): %s\n", rc, hh_last_error(nullptr));
             std::abort();
         }
+        const char *tab = std::getenv("HERMITE_HIP_TUNING_FILE");
+        if (tab && *tab) check(hh_load_tuning_file(ctx, tab), "hh_load_tuning_file");
     }
     ~Gravity() { hh_destroy(ctx); }
     Gravity(const Gravity &) = delete;

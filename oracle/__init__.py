@@ -282,6 +282,21 @@ class RefAVX:
             self._run(["pot", fin, len(P), repr(float(eps2)), fout])
             return np.fromfile(fout)
 
+    def run(self, P, eps2, eta, eta_s, dtmax, tend, threads=None):
+        """The reference driver on the AVX2 back end from t = 0 (init_step,
+        then integrate_one_dtmax until tend; only the integration is timed).
+        Returns {e0, e1, de, num_step, num_bstep, seconds, threads}."""
+        import tempfile
+
+        env = dict(os.environ)
+        if threads:
+            env["OMP_NUM_THREADS"] = str(threads)
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        with tempfile.TemporaryDirectory() as td:
+            fin = os.path.join(td, "p.bin")
+            P.tofile(fin)
+            return self._run(["run", fin, len(P)] + [repr(float(a)) for a in (eps2, eta, eta_s, dtmax, tend)], env)
+
 
 class Ref:
     """The reference's generic FP64 back end (SRC/hermite{4,6,8}.h) and its

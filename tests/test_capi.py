@@ -13,22 +13,25 @@ import hermite_amd as H
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "hermite_hip.h")
+DRIVER_HEADER = os.path.join(ROOT, "include", "hermite_driver.h")
 
 
-def declared():
-    text = open(HEADER).read()
+def declared(path):
+    text = open(path).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(hh_[a-z_0-9]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(hh[rd]?_[a-z_0-9]+)\s*\(", text)))
 
 
-def test_every_declared_symbol_is_exported():
-    names = declared()
-    assert len(names) >= 12
-    out = subprocess.run(["nm", "-D", "--defined-only", H.HIP_LIB], capture_output=True, text=True, check=True).stdout
-    exported = set(re.findall(r"\bT (hh_[a-z_0-9]+)\b", out))
+@pytest.mark.parametrize("header,lib,minimum", [(HEADER, "hip", 30), (DRIVER_HEADER, "host", 18)])
+def test_every_declared_symbol_is_exported(header, lib, minimum):
+    path = H.HIP_LIB if lib == "hip" else H.HOST_LIB
+    names = declared(header)
+    assert len(names) >= minimum
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (hh[rd]?_[a-z_0-9]+)\b", out))
     missing = [n for n in names if n not in exported]
     assert not missing, missing
-    L = H.hip_lib()
+    L = H.hip_lib() if lib == "hip" else H.host_lib()
     for n in names:
         assert getattr(L, n) is not None
 

@@ -236,3 +236,36 @@ def test_both_kernels_all_small_nact(order, variant, tol):
             assert errs[0].max() <= tol and errs[1].max() <= tol, (kernel, nact, errs[0].max(), errs[1].max())
             assert np.max(np.abs(pot - potref[:nact]) / np.abs(potref[:nact])) <= tol
     g.close()
+
+
+@pytest.mark.parametrize("variant", ["fp64", "split"])
+def test_tuning_table_selects_geometry(variant, tmp_path):
+    """A tuning table row picks kernel, i per lane and j-split for its nact
+    range; every geometry gives the same forces (FP64 to 1e-12, split to its
+    own bound), and a text table loads the same way."""
+    order, n = 6, 4096
+    print(f"tuning table {variant}", flush=True)
+    P = hermite_state(order, n, seed=5)
+    eps2 = (1 / 256) ** 2
+    g = H.Gravity(order, n, variant, tuning=False)
+    g.set_jp_all(P)
+    g.predict_all(0.01)
+    base = {nact: g.calc_force_on_first_nact(nact, eps2) for nact in (3, 200, 1500)}
+    rows = [(16, 2, 1, 8), (512, 1, 2, 4), (2**31 - 1, 1, 1, 3)]
+    g.set_tuning_table(rows)
+    tol = 1e-12 if variant == "fp64" else 5e-6
+    for nact, (k, s) in {3: (2, 8), 200: (1, 4), 1500: (1, 3)}.items():
+        F = g.calc_force_on_first_nact(nact, eps2)
+        geo = g.last_geometry()
+        assert (geo["kernel"], geo["jsplit"]) == (k, s), (nact, geo)
+        assert np.abs(F - base[nact]).max() <= tol * np.abs(base[nact]).max()
+    path = tmp_path / "t.txt"
+    from hermite_amd.autotune import write_text
+    write_text([(2**31 - 1, 1, 1, 2)], str(path))
+    g.load_tuning_file(str(path))
+    g.calc_force_on_first_nact(200, eps2)
+    assert g.last_geometry()["jsplit"] == 2
+    g.set_tuning(1, 8)  # explicit override wins over the table
+    g.calc_force_on_first_nact(200, eps2)
+    assert g.last_geometry()["jsplit"] == 8
+    g.close()

@@ -152,3 +152,26 @@ def test_ensemble_fp64_matches_oracle_statistically():
         assert abs(st["num_step"] / o.num_step - 1) < 0.02
         assert abs((ens.energy(k) - e0) / e0) < 3 * abs((o.energy() - e0) / e0) + 1e-12
     ens.close()
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_resident_force_timing_changes_nothing(use_graph):
+    """Event timing around every force launch (event record nodes inside the
+    graph) leaves the integration bit-identical and counts every launch."""
+    order, n, eps2 = 6, 500, (1 / 32) ** 2
+    o = initialised(order, n, 43, eps2, 0.4, 0.1, 1 / 16)
+    P0 = o.ptcl()
+    print(f"resident timing graph={use_graph}", flush=True)
+    a = H.Resident(order, "fp64", [(P0, eps2, 0.4, 1 / 16, 0.0, 0.0)])
+    a.run(1 / 8, use_graph=use_graph, batch=8)
+    b = H.Resident(order, "fp64", [(P0, eps2, 0.4, 1 / 16, 0.0, 0.0)])
+    b.set_timing(True)
+    b.run(1 / 16, use_graph=use_graph, batch=8)
+    b.run(1 / 8, use_graph=use_graph, batch=8)
+    ms, launches = b.force_timing()
+    assert a.stats() == b.stats()
+    assert np.array_equal(a.particles(), b.particles())
+    assert launches == b.steps_launched and launches >= b.stats()["num_bstep"]
+    assert 0 < ms < 1e4
+    a.close()
+    b.close()
