@@ -418,7 +418,8 @@ class Resident:
         self._check(hip_lib().hhr_set_plan(self.h, jpar_max, smax, task_target, jpar_target), "hhr_set_plan")
 
     def set_timing(self, on=True):
-        """Bracket every force launch with HIP events (resets the totals)."""
+        """Bracket every force launch with HIP events (resets the totals);
+        timed runs launch their steps without a graph."""
         self._check(hip_lib().hhr_set_timing(self.h, int(on)), "hhr_set_timing")
 
     def force_timing(self):

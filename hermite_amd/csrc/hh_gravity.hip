@@ -642,8 +642,9 @@ struct hhr_ctx {
     double last_run_ms = 0.0;
     long long steps_launched = 0;
     // Optional per-launch timing of the force kernel: an event pair around
-    // every force launch of a batch (event record nodes inside a graph).
-    bool timing = false, capturing = false;
+    // every force launch of a batch.  Timed runs launch their steps directly
+    // on the stream (no graph), so the events are ordinary stream events.
+    bool timing = false;
     std::vector<hipEvent_t> kev;
     double force_ms = 0.0;
     long long force_launches = 0;
@@ -684,20 +685,16 @@ static RArgs r_args(hhr_ctx *c)
 
 static hipError_t r_mark(hhr_ctx *c, int k)
 {
-    if (!c->timing) return hipSuccess;
-    return c->capturing ? hipEventRecordWithFlags(c->kev[k], c->stream, hipEventRecordExternal)
-                        : hipEventRecord(c->kev[k], c->stream);
+    return c->timing ? hipEventRecord(c->kev[k], c->stream) : hipSuccess;
 }
 
-// One block step: six launches; s is the step's index within its batch.
+// One block step: four launches; s is the step's index within its batch.
 template <int ORDER, int MODE>
 static hipError_t r_launch_step(hhr_ctx *c, int s)
 {
     const RArgs a = r_args(c);
-    hipLaunchKernelGGL(r_min_kernel, dim3(c->ntiles), dim3(NT), 0, c->stream, a);
-    hipLaunchKernelGGL(r_compact_kernel, dim3(c->ntiles), dim3(NT), 0, c->stream, a);
+    hipLaunchKernelGGL((r_compact_kernel<ORDER, MODE == EXACT>), dim3(c->ntiles), dim3(NT), 0, c->stream, a);
     hipLaunchKernelGGL((r_plan_kernel<ORDER>), dim3(1), dim3(NT), 0, c->stream, a);
-    hipLaunchKernelGGL((r_predict_kernel<ORDER, MODE == EXACT>), dim3(c->ntiles), dim3(NT), 0, c->stream, a);
     hipError_t e = r_mark(c, 2 * s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((r_force_kernel<ORDER, MODE>), dim3(c->force_grid), dim3(NT), 0, c->stream, a);
@@ -842,7 +839,16 @@ int hhr_create(int order, int variant, int device, int nclu, const int *ns, cons
         h.tsys = pr[4];
         h.tend = pr[4];
         h.tnext = pr[4];
-        h.tmin_bits = 0x7ff0000000000000ULL;
+        // first block time; the minima are atomicMin on the bit patterns,
+        // which order like the values only for positive doubles
+        double tmin = INFINITY;
+        for (int i = 0; i < nk; i++) tmin = std::min(tmin, tn[(size_t)o + i]);
+        if (!(tmin > 0.0) || !std::isfinite(tmin) || pr[4] < 0.0) {
+            delete c;
+            return rfail(nullptr, HH_E_ARG, "block times tlast + dt must be positive and finite, tsys >= 0");
+        }
+        for (int q = 0; q < R_SLOTS; q++) h.tmin_slot[0][q] = h.tmin_slot[1][q] = 0x7ff0000000000000ULL;
+        std::memcpy(&h.tmin_slot[0][0], &tmin, sizeof(double));
         h.off = o;
         h.n = nk;
         h.ntiles = ntk;
@@ -862,10 +868,11 @@ int hhr_create(int order, int variant, int device, int nclu, const int *ns, cons
     };
     hipError_t e;
     // Partials of one block step: i-parallel clusters hold nslice * ngrp <=
-    // budget + ngrp tasks of NT i each, budgets summing to at most
-    // task_target + nclu, and ngrp summing to at most (tot + NT nclu) / NT;
+    // budget + ngrp tasks of G = NT * r_ni i each, budgets summing to at
+    // most task_target + nclu, and ngrp summing to at most (tot + G nclu) / G;
     // j-parallel clusters need less.  Sized for the largest task target.
-    c->part_cap = (size_t)F * (tot + (size_t)NT * nclu) + (size_t)F * NT * 2 * (R_TASK_MAX + nclu);
+    const size_t G = (size_t)NT * r_ni(FP64);
+    c->part_cap = (size_t)F * (tot + G * nclu) + (size_t)F * G * (R_TASK_MAX + nclu);
     if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) return bail("stream", e);
     if ((e = hipEventCreate(&c->ev0)) != hipSuccess) return bail("event", e);
     if ((e = hipEventCreate(&c->ev1)) != hipSuccess) return bail("event", e);
@@ -902,9 +909,11 @@ int hhr_destroy(hhr_ctx *c)
 // launched in batches of batch_steps (captured once into a hipGraph when
 // use_graph != 0); after each batch the number of clusters still active is
 // read back.  Extra steps after the last cluster finishes are no-ops.
+// With force timing on, steps are launched directly (use_graph ignored).
 int hhr_run(hhr_ctx *c, double tend, int use_graph, int batch_steps)
 {
     if (!c || batch_steps <= 0) return HH_E_ARG;
+    if (c->timing) use_graph = 0;
     RCHK(c, hipSetDevice(c->device));
     // set tend for every cluster
     std::vector<RClu> hc(c->nclu);
@@ -912,10 +921,6 @@ int hhr_run(hhr_ctx *c, double tend, int use_graph, int batch_steps)
     for (auto &h : hc) h.tend = tend;
     RCHK(c, hipMemcpy(c->clu, hc.data(), sizeof(RClu) * c->nclu, hipMemcpyHostToDevice));
     if (c->timing && (int)c->kev.size() != 2 * batch_steps) {
-        if (c->gexec) RCHK(c, hipGraphExecDestroy(c->gexec));
-        if (c->graph) RCHK(c, hipGraphDestroy(c->graph));
-        c->gexec = nullptr;
-        c->graph = nullptr;
         r_free_events(c);
         c->kev.assign(2 * batch_steps, nullptr);
         for (auto &e : c->kev) RCHK(c, hipEventCreate(&e));
@@ -926,10 +931,8 @@ int hhr_run(hhr_ctx *c, double tend, int use_graph, int batch_steps)
         c->gexec = nullptr;
         c->graph = nullptr;
         RCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        c->capturing = true;
         hipError_t e = hipSuccess;
         for (int s = 0; s < batch_steps && e == hipSuccess; s++) e = r_step(c, s);
-        c->capturing = false;
         hipGraph_t gph = nullptr;
         const hipError_t e2 = hipStreamEndCapture(c->stream, &gph);
         if (e != hipSuccess) return rfail(c, HH_E_HIP, std::string("step capture: ") + hipGetErrorString(e));
@@ -1010,10 +1013,6 @@ int hhr_set_timing(hhr_ctx *c, int on)
     if (c->timing != (on != 0)) {
         RCHK(c, hipSetDevice(c->device));
         RCHK(c, hipStreamSynchronize(c->stream));
-        if (c->gexec) RCHK(c, hipGraphExecDestroy(c->gexec));
-        if (c->graph) RCHK(c, hipGraphDestroy(c->graph));
-        c->gexec = nullptr;
-        c->graph = nullptr;
         r_free_events(c);
     }
     c->timing = on != 0;

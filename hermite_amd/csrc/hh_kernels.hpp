@@ -414,6 +414,8 @@ __global__ __launch_bounds__(NT) void tile_kernel(TileArgs a)
         for (int w = 0; w < Q; w++) jr[w] = a.src[(size_t)w * npad + t0 * TJ + tid];
     }
     const float eps2f = (float)a.eps2;
+    // a wave whose lanes are all past the i range only helps stage tiles
+    const bool busy = iblock * NT * NI + (tid & ~63) < a.nact;
 
     for (int t = t0; t < t1; t++) {
         __syncthreads();
@@ -424,6 +426,7 @@ __global__ __launch_bounds__(NT) void tile_kernel(TileArgs a)
 #pragma unroll
             for (int w = 0; w < Q; w++) jr[w] = a.src[(size_t)w * npad + (t + 1) * TJ + tid];
         }
+        if (!busy) continue;
         const int jbase = t * TJ;
         if constexpr (MODE == EXACT) {
             const int jn = min(TJ, a.nbody - jbase);
@@ -487,8 +490,10 @@ __global__ __launch_bounds__(NT) void tile_kernel(TileArgs a)
     const size_t ipad = (size_t)a.ipad;
 #pragma unroll
     for (int k = 0; k < NI; k++)
+        if (ii[k] < a.nact) {
 #pragma unroll
-        for (int c = 0; c < NC; c++) a.part[((size_t)slice * NC + c) * ipad + ii[k]] = A[k][c];
+            for (int c = 0; c < NC; c++) a.part[((size_t)slice * NC + c) * ipad + ii[k]] = A[k][c];
+        }
 }
 
 // Sum of the j-slice partials part[(s*nc + c)*ipad + i] over s, for i in

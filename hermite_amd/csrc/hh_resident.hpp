@@ -4,17 +4,24 @@
 // hipGraph) and no host round trip.
 //
 // Per block step:
-//   r_min_kernel      per-cluster next block time min_i(tlast_i + dt_i)
-//   r_compact_kernel  active list of every cluster (wave-aggregated append)
+//   r_compact_kernel  active list of every cluster (wave-aggregated append),
+//                     prediction of every slot of the active clusters to the
+//                     block time, and the minimum block time of the slots
+//                     that stay inactive
 //   r_plan_kernel     per-cluster dtlim, step counters, kernel choice and
 //                     geometry, task table of the persistent force kernel
-//   r_predict_kernel  prediction of every particle of the active clusters
 //   r_force_kernel    persistent: workgroups pull (cluster, i-group, j-slice)
 //                     tasks from a device queue and write partial forces
 //   r_reduce_correct_kernel
-//                     sums each active particle's slices in a fixed order and
+//                     sums each active particle's slices in a fixed order,
 //                     applies the Hermite corrector (SRC/hermite{4,6,8}.h
-//                     correct) to it in place.
+//                     correct) to it in place and folds its new block time
+//                     into the next step's minimum.
+// The next block time of a cluster, min_i(tlast_i + dt_i), is therefore
+// complete when a step ends: min over the slots left inactive (compact) and
+// over the new times of the corrected ones (reduce_correct).  The minimum and
+// the active count live in two parity buffers (RGlob::step & 1 selects the
+// current one), so no kernel has to clear what another is still reading.
 // Particles are not permuted (the reference sorts them by dt level,
 // SRC/nbodysystem.h:505-555): the active set is taken by comparison, and j
 // is always summed in the particles' original order.
@@ -30,7 +37,11 @@ constexpr int R_DT = 26, R_ID = 27, R_D6A = 28, R_D7A = 31;
 __host__ __device__ constexpr int r_words(int o) { return o == 8 ? 34 : 28; }
 
 constexpr int R_GI = 4;          // i per group of the j-parallel task
+// i per lane of the i-parallel task: two i share every LDS read of a j
+// (halving LDS traffic per interaction); the EXACT variant keeps one
+__host__ __device__ constexpr int r_ni(int mode) { return mode == EXACT ? 1 : 2; }
 constexpr int R_TASK_MAX = 8192; // largest task target a plan may ask for
+constexpr int R_SLOTS = 32;      // words a block-time minimum is spread over
 
 // Plan parameters (hhr_set_plan): active counts up to jpar_max use
 // j-parallel groups; otherwise i-parallel groups of NT with at most smax
@@ -44,8 +55,11 @@ struct RPlan {
 struct RClu {
     double tsys, tend, dtmax, dtmin, eps2, etapow;
     double tnext, dtlim;
-    unsigned long long tmin_bits;
+    // block-time minimum, parity buffered and spread over R_SLOTS words so
+    // that the atomics of one step do not all hit one address
+    unsigned long long tmin_slot[2][R_SLOTS];
     long long num_step, num_bstep, dtmin_hits;
+    int acnt[2];         // active-list append counters, parity buffered
     int off, n, ntiles;  // particles [off, off + n), tiles from off / TJ
     int nact;
     int mode;            // 0 idle, 1 i-parallel tiles, 2 j-parallel groups
@@ -58,7 +72,7 @@ struct RGlob {
     int next_task, total_tasks, nclu, nactive, exact;
     int error;     // a cluster's block time stopped advancing
     int total_active;
-    int pad;
+    int step;      // block steps planned; step & 1 selects the current buffers
     RPlan plan;
 };
 
@@ -206,7 +220,7 @@ __device__ void r_correct(const RPart &p, const double *f, double etapow, double
 
 // Correct particle i with force f, clamp dt to dtmin, refresh tn[i].
 template <int ORDER>
-__device__ void r_finish(const RArgs &a, const RClu &c, int i, const double *f)
+__device__ double r_finish(const RArgs &a, const RClu &c, int i, const double *f)
 {
     const RPart p{a.rp, a.npad, i};
     r_correct<ORDER>(p, f, c.etapow, c.dtlim);
@@ -216,10 +230,12 @@ __device__ void r_finish(const RArgs &a, const RClu &c, int i, const double *f)
         p.w(R_DT) = dt;
         atomicAdd((unsigned long long *)&a.clu[a.tile_clu[i / TJ]].dtmin_hits, 1ULL);
     }
-    a.tn[i] = p.w(7) + dt;
+    const double tn = p.w(7) + dt;
+    a.tn[i] = tn;
     // a non-finite state (a step that ran away on noisy derivatives) stops
     // the run instead of integrating garbage
     if (!isfinite(p.w(0) + p.w(1) + p.w(2) + p.w(4) + p.w(5) + p.w(6))) atomicExch(&a.glob->error, 2);
+    return tn;
 }
 
 // ------------------------------------------------------------ schedule
@@ -230,44 +246,61 @@ __device__ __forceinline__ double r_wave_min(double x)
     return x;
 }
 
-// One workgroup per 256-slot tile; a tile belongs to one cluster.
-__global__ __launch_bounds__(NT) void r_min_kernel(RArgs a)
+__device__ __forceinline__ unsigned long long r_bits(double t) { return (unsigned long long)__double_as_longlong(t); }
+
+// Current block time of a cluster: the minimum over its slots, in every
+// lane of the calling wave.
+__device__ __forceinline__ double r_tmin_wave(const RClu &c, int p)
 {
+    const int lane = threadIdx.x & 63;
+    const double t = lane < R_SLOTS ? __longlong_as_double((long long)c.tmin_slot[p][lane]) : r_inf();
+    return r_wave_min(t);
+}
+
+// One workgroup per 256-slot tile; a tile belongs to one cluster.  Times
+// are positive, and positive doubles order like their bit patterns, so the
+// minima are integer atomicMin on the bits.
+template <int ORDER, bool EX>
+__global__ __launch_bounds__(NT) void r_compact_kernel(RArgs a)
+{
+    constexpr int W = jp_words(ORDER), Q = pr_words(ORDER);
     __shared__ double wmin[NT / 64];
     const int tile = blockIdx.x, tid = threadIdx.x;
     const int k = a.tile_clu[tile];
-    const RClu &c = a.clu[k];
-    if (tile * TJ == c.off && tid == 0) a.clu[k].nact = 0;  // first tile of the cluster
-    double t = a.tn[(size_t)tile * TJ + tid];
-    t = r_wave_min(t);
-    if ((tid & 63) == 0) wmin[tid >> 6] = t;
+    RClu &c = a.clu[k];
+    const int p = a.glob->step & 1;
+    const double tmin = r_tmin_wave(c, p);
+    const bool live = tmin <= c.tend;
+    const int i = tile * TJ + tid;
+    const double t = a.tn[i];
+    const bool act = live && t == tmin;
+    const unsigned long long mask = __ballot(act);
+    const int lane = tid & 63;
+    if (mask != 0) {
+        const int leader = __ffsll((long long)mask) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&c.acnt[p], __popcll(mask));
+        base = __shfl(base, leader, 64);
+        if (act) {
+            const unsigned long long below = lane == 0 ? 0ULL : (mask & ((~0ULL) >> (64 - lane)));
+            a.ilist[c.off + base + __popcll(below)] = i;
+        }
+    }
+    // minimum block time of the slots this step leaves alone
+    double m = r_wave_min(act ? r_inf() : t);
+    if (lane == 0) wmin[tid >> 6] = m;
+    if (live) {
+        double r[W], q[Q];
+#pragma unroll
+        for (int w = 0; w < W; w++) r[w] = a.rp[(size_t)w * a.npad + i];
+        predict_words<ORDER, EX>(r, tmin, q);
+#pragma unroll
+        for (int w = 0; w < Q; w++) a.jpr[(size_t)w * a.npad + i] = q[w];
+    }
     __syncthreads();
     if (tid == 0) {
-        double m = wmin[0];
         for (int w = 1; w < NT / 64; w++) m = fmin(m, wmin[w]);
-        // positive doubles order like their bit patterns
-        atomicMin(&a.clu[k].tmin_bits, (unsigned long long)__double_as_longlong(m));
-    }
-}
-
-__global__ __launch_bounds__(NT) void r_compact_kernel(RArgs a)
-{
-    const int tile = blockIdx.x, tid = threadIdx.x;
-    const int k = a.tile_clu[tile];
-    RClu &c = a.clu[k];
-    const double tmin = __longlong_as_double((long long)c.tmin_bits);
-    const int i = tile * TJ + tid;
-    const bool act = (tmin <= c.tend) && a.tn[i] == tmin;
-    const unsigned long long mask = __ballot(act);
-    if (mask == 0) return;
-    const int lane = tid & 63;
-    const int leader = __ffsll((long long)mask) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(&c.nact, __popcll(mask));
-    base = __shfl(base, leader, 64);
-    if (act) {
-        const unsigned long long below = lane == 0 ? 0ULL : (mask & ((~0ULL) >> (64 - lane)));
-        a.ilist[c.off + base + __popcll(below)] = i;
+        if (m < r_inf()) atomicMin(&c.tmin_slot[p ^ 1][tile % R_SLOTS], r_bits(m));
     }
 }
 
@@ -297,9 +330,12 @@ __global__ __launch_bounds__(NT) void r_plan_kernel(RArgs a)
     __shared__ double s_total;
     const int tid = threadIdx.x;
     const int nclu = a.glob->nclu;
+    const int p = a.glob->step & 1;
     double w = 0.0;
     for (int k = tid; k < nclu; k += NT) {
         RClu &c = a.clu[k];
+        c.nact = c.acnt[p];
+        c.acnt[p] = 0;
         c.mode = 0;
         c.ntask = 0;
         if (c.nact > 0) w += (double)c.nact * c.n;
@@ -315,8 +351,11 @@ __global__ __launch_bounds__(NT) void r_plan_kernel(RArgs a)
     const double total = s_total;
     for (int k = tid; k < nclu; k += NT) {
         RClu &c = a.clu[k];
-        const double tmin = __longlong_as_double((long long)c.tmin_bits);
-        c.tmin_bits = (unsigned long long)__double_as_longlong(r_inf());
+        double tmin = r_inf();
+        for (int q = 0; q < R_SLOTS; q++) {
+            tmin = fmin(tmin, __longlong_as_double((long long)c.tmin_slot[p][q]));
+            c.tmin_slot[p][q] = r_bits(r_inf());
+        }
         if (c.nact == 0) continue;
         const double dtlim = r_dtlim(tmin, c.dtmax);
         if (!(tmin > c.tsys) || dtlim <= 0.0) {  // collapsed step: stop this cluster
@@ -335,7 +374,7 @@ __global__ __launch_bounds__(NT) void r_plan_kernel(RArgs a)
         if (a.glob->exact || c.nact > pl.jpar_max) {
             const int budget = max(1, (int)ceil(pl.task_target * share));
             c.mode = 1;
-            c.ngrp = (c.nact + NT - 1) / NT;
+            c.ngrp = (c.nact + NT * r_ni(a.glob->exact ? EXACT : FP64) - 1) / (NT * r_ni(a.glob->exact ? EXACT : FP64));
             int ns = a.glob->exact ? 1 : min(min(c.ntiles, pl.smax), max(1, (budget + c.ngrp - 1) / c.ngrp));
             c.tps = (c.ntiles + ns - 1) / ns;
             c.nslice = (c.ntiles + c.tps - 1) / c.tps;
@@ -363,7 +402,7 @@ __global__ __launch_bounds__(NT) void r_plan_kernel(RArgs a)
             if (c.mode != 0) {
                 nactive++;
                 act += c.nact;
-                c.ipad = c.ngrp * (c.mode == 1 ? NT : R_GI);
+                c.ipad = c.ngrp * (c.mode == 1 ? NT * r_ni(a.glob->exact ? EXACT : FP64) : R_GI);
                 p0 += (long long)c.nslice * NC * c.ipad;
             }
         }
@@ -371,24 +410,8 @@ __global__ __launch_bounds__(NT) void r_plan_kernel(RArgs a)
         a.glob->next_task = 0;
         a.glob->nactive = nactive;
         a.glob->total_active = act;
+        a.glob->step += 1;
     }
-}
-
-// Predict every slot of every active cluster to its tnext.
-template <int ORDER, bool EX>
-__global__ __launch_bounds__(NT) void r_predict_kernel(RArgs a)
-{
-    constexpr int W = jp_words(ORDER), Q = pr_words(ORDER);
-    const int tile = blockIdx.x;
-    const RClu &c = a.clu[a.tile_clu[tile]];
-    if (c.mode == 0) return;
-    const int i = tile * TJ + threadIdx.x;
-    double r[W], q[Q];
-#pragma unroll
-    for (int w = 0; w < W; w++) r[w] = a.rp[(size_t)w * a.npad + i];
-    predict_words<ORDER, EX>(r, c.tnext, q);
-#pragma unroll
-    for (int w = 0; w < Q; w++) a.jpr[(size_t)w * a.npad + i] = q[w];
 }
 
 // Cluster of a task: the last cluster whose first task is <= task.
@@ -403,7 +426,11 @@ __device__ int r_find_cluster(const RClu *clu, int nclu, int task)
     return lo;
 }
 
-// Persistent force + correct kernel.
+// Persistent force kernel: workgroups pull tasks of either kind from one
+// queue.  (Measured: one kernel with both task kinds runs the tile tasks
+// faster than a tile-only kernel with fewer registers and twice the waves
+// per SIMD; the j-parallel path's register budget leaves the tile loop two
+// waves per SIMD, which suits its LDS traffic.)
 template <int ORDER, int MODE>
 __global__ __launch_bounds__(NT) void r_force_kernel(RArgs a)
 {
@@ -438,15 +465,25 @@ __global__ __launch_bounds__(NT) void r_force_kernel(RArgs a)
         const double *src = a.jpr;
 
         if (c.mode == 1) {
-            // ---- i in lanes, j-tiles of the slice through LDS
-            const int ii = grp * NT + tid;
-            const int gi = a.ilist[c.off + min(ii, c.nact - 1)];
-            double pi[Q];
+            // ---- i in lanes (NI per lane), j-tiles of the slice through LDS
+            constexpr int NI = r_ni(MODE), GS = NT * NI;
+            int ii[NI];
+            double pi[NI][Q];
 #pragma unroll
-            for (int w = 0; w < Q; w++) pi[w] = src[(size_t)w * npad + gi];
-            double A[NC];
+            for (int k = 0; k < NI; k++) {
+                ii[k] = grp * GS + k * NT + tid;
+                const int gi = a.ilist[c.off + min(ii[k], c.nact - 1)];
 #pragma unroll
-            for (int q = 0; q < NC; q++) A[q] = 0.0;
+                for (int w = 0; w < Q; w++) pi[k][w] = src[(size_t)w * npad + gi];
+            }
+            // a wave whose lanes are all past the active list only helps
+            // stage the tiles; its SIMD slots go to the other waves
+            const bool busy = grp * GS + (tid & ~63) < c.nact;
+            double A[NI][NC];
+#pragma unroll
+            for (int k = 0; k < NI; k++)
+#pragma unroll
+                for (int q = 0; q < NC; q++) A[k][q] = 0.0;
             const int tb = c.off / TJ;
             const int t0 = slice * c.tps, t1 = min(t0 + c.tps, c.ntiles);
             double jr[Q];
@@ -461,14 +498,14 @@ __global__ __launch_bounds__(NT) void r_force_kernel(RArgs a)
 #pragma unroll
                     for (int w = 0; w < Q; w++) jr[w] = src[(size_t)w * npad + (tb + t + 1) * TJ + tid];
                 }
-                const int jbase = c.off + t * TJ;
+                if (!busy) continue;
                 if constexpr (MODE == EXACT) {
-                    const int jn = min(TJ, c.off + c.n - jbase);
+                    const int jn = min(TJ, c.off + c.n - (c.off + t * TJ));
                     for (int jj = 0; jj < jn; jj++) {
                         double q[L];
 #pragma unroll
                         for (int w = 0; w < L; w++) q[w] = sj[jj * L + w];
-                        body_exact<ORDER, false>(pi, q, eps2, false, 0, A);
+                        body_exact<ORDER, false>(pi[0], q, eps2, false, 0, A[0]);
                     }
                 } else if constexpr (MODE == FP64) {
 #pragma unroll 2
@@ -476,31 +513,40 @@ __global__ __launch_bounds__(NT) void r_force_kernel(RArgs a)
                         double q[L];
 #pragma unroll
                         for (int w = 0; w < L; w++) q[w] = sj[jj * L + w];
-                        body_fp64<ORDER, false>(pi, q, eps2, false, 0, A);
+#pragma unroll
+                        for (int k = 0; k < NI; k++) body_fp64<ORDER, false>(pi[k], q, eps2, false, 0, A[k]);
                     }
                 } else {
                     for (int jb = 0; jb < TJ; jb += FOLD) {
-                        float Af[NC];
+                        float Af[NI][NC];
 #pragma unroll
-                        for (int q = 0; q < NC; q++) Af[q] = 0.0f;
+                        for (int k = 0; k < NI; k++)
+#pragma unroll
+                            for (int q = 0; q < NC; q++) Af[k][q] = 0.0f;
 #pragma unroll 2
                         for (int jj = jb; jj < jb + FOLD; jj++) {
                             double q[L];
 #pragma unroll
                             for (int w = 0; w < L; w++) q[w] = sj[jj * L + w];
-                            body_split<ORDER, false>(pi, q, eps2f, false, 0, Af);
+#pragma unroll
+                            for (int k = 0; k < NI; k++) body_split<ORDER, false>(pi[k], q, eps2f, false, 0, Af[k]);
                         }
 #pragma unroll
-                        for (int q = 0; q < NC; q++) A[q] += (double)Af[q];
+                        for (int k = 0; k < NI; k++)
+#pragma unroll
+                            for (int q = 0; q < NC; q++) A[k][q] += (double)Af[k][q];
                     }
                 }
             }
             double *part = a.part + c.pofs;
 #pragma unroll
-            for (int q = 0; q < NC; q++) part[((size_t)slice * NC + q) * c.ipad + ii] = A[q];
-        } else {
+            for (int k = 0; k < NI; k++)
+                if (ii[k] < c.nact) {
+#pragma unroll
+                    for (int q = 0; q < NC; q++) part[((size_t)slice * NC + q) * c.ipad + ii[k]] = A[k][q];
+                }
+        } else if constexpr (MODE != EXACT) {  // EXACT always plans mode 1
             // ---- j in lanes, R_GI i-particles per group
-            if constexpr (MODE == EXACT) continue;  // EXACT always plans mode 1
             int gi[R_GI];
             double pi[R_GI][Q];
 #pragma unroll
@@ -580,6 +626,11 @@ __global__ __launch_bounds__(NT) void r_reduce_correct_kernel(RArgs a)
     constexpr int PB = NT / RL;  // particles per workgroup pass
     const int nclu = a.glob->nclu, total = a.glob->total_active;
     const int l = threadIdx.x % RL;
+    const int nb = a.glob->step & 1;  // the plan has advanced the step: next buffers
+    // each lane keeps the minimum of the new block times it produced for
+    // one cluster and flushes it when the cluster changes
+    double tacc = r_inf();
+    int kacc = -1;
     for (int base = blockIdx.x * PB; base < total; base += gridDim.x * PB) {
         const int pos = base + threadIdx.x / RL;
         const bool live = pos < total;
@@ -598,7 +649,50 @@ __global__ __launch_bounds__(NT) void r_reduce_correct_kernel(RArgs a)
         }
 #pragma unroll
         for (int q = 0; q < NC; q++) S[q] = lane_tree8(S[q]);
-        if (live && l == 0) r_finish<ORDER>(a, c, a.ilist[c.off + ii], S);
+        if (live && l == 0) {
+            const double tn = r_finish<ORDER>(a, c, a.ilist[c.off + ii], S);
+            if (kacc != k && kacc >= 0) atomicMin(&a.clu[kacc].tmin_slot[nb][blockIdx.x % R_SLOTS], r_bits(tacc));
+            if (kacc != k) tacc = r_inf();
+            kacc = k;
+            tacc = fmin(tacc, tn);
+        }
+    }
+    // one atomic per workgroup when all its lanes worked for one cluster
+    __shared__ int s_k[NT / 64];
+    __shared__ double s_t[NT / 64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int kref = __shfl(kacc, 0, 64);
+    const bool wuni = __all(kacc < 0 || kacc == kref);
+    if (wuni) {
+        const double m = r_wave_min(tacc);
+        if (lane == 0) {
+            s_k[wave] = kref;
+            s_t[wave] = m;
+        }
+    } else {
+        if (lane == 0) s_k[wave] = -2;
+        if (kacc >= 0) atomicMin(&a.clu[kacc].tmin_slot[nb][blockIdx.x % R_SLOTS], r_bits(tacc));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int kb = -1;
+        bool buni = true;
+        for (int w = 0; w < NT / 64; w++) {
+            if (s_k[w] == -2) continue;  // flushed by its lanes
+            if (s_k[w] < 0) continue;    // no particles
+            if (kb < 0) kb = s_k[w];
+            else if (s_k[w] != kb) buni = false;
+        }
+        if (buni) {
+            double m = r_inf();
+            for (int w = 0; w < NT / 64; w++)
+                if (s_k[w] >= 0) m = fmin(m, s_t[w]);
+            if (kb >= 0 && m < r_inf()) atomicMin(&a.clu[kb].tmin_slot[nb][blockIdx.x % R_SLOTS], r_bits(m));
+        } else {
+            for (int w = 0; w < NT / 64; w++)
+                if (s_k[w] >= 0 && s_t[w] < r_inf())
+                    atomicMin(&a.clu[s_k[w]].tmin_slot[nb][blockIdx.x % R_SLOTS], r_bits(s_t[w]));
+        }
     }
 }
 

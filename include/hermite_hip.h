@@ -185,15 +185,15 @@ long long hhr_steps_launched(const hhr_ctx *ctx);
  * active counts up to jpar_max use j-parallel groups of 4 i; larger ones
  * i-parallel groups of 256 with at most smax j-slices; a block step aims at
  * task_target i-parallel (jpar_target j-parallel) tasks over all clusters
-  * (both at most 8192).  Defaults 64, 256, 4096, 256 (measured best for a
+ * (both at most 8192).  Defaults 64, 256, 4096, 256 (measured best for a
  * 6th-order N = 65536 Plummer sphere). */
 int hhr_set_plan(hhr_ctx *ctx, int jpar_max, int smax, int task_target, int jpar_target);
 
-/* Force-kernel timing (no reference counterpart): when on, every force
- * launch of hhr_run is bracketed by HIP events on the context's stream
- * (event record nodes inside the graph) and hhr_force_timing reports the
- * summed device milliseconds and the number of launches since timing was
- * switched on. */
+/* Force-kernel timing (no reference counterpart): when on, hhr_run launches
+ * its steps directly on the context's stream (no graph; the kernels are the
+ * same) with a HIP event pair around every force launch, and
+ * hhr_force_timing reports the summed device milliseconds and the number of
+ * launches since timing was switched on. */
 int hhr_set_timing(hhr_ctx *ctx, int on);
 int hhr_force_timing(const hhr_ctx *ctx, double *ms_total, long long *launches);
 /* Reference Particle records of cluster k (ns[k] * hh_particle_words). */

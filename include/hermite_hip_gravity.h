@@ -13,7 +13,9 @@
  * The class keeps the interface of the TITAN back end
  * (SRC/hermite6-titan.h:4-133): GParticle / GPredictor types (their sizes
  * feed the driver's bandwidth print, SRC/nbodysystem.h:152-153), set_jp,
- * predict_all, calc_force_on_first_nact, calc_potential.  Like the
+ * predict_all, calc_force_on_first_nact, calc_potential, and the
+ * FAST_OMP_SYNC pair predict_all_fast_omp / calc_force_on_first_nact_fast_omp
+ * with NACT_PARALLEL_THRESH.  Like the
  * reference's checkCudaErrors (SRC/cuda_pointer.h:8-9) it aborts on a device
  * error, after printing the library's message.
  *
@@ -39,6 +41,10 @@
 
 struct Gravity {
     enum {
+        // FAST_OMP_SYNC drivers correct nact particles with an omp for
+        // above this count, on the master thread below it
+        // (SRC/nbodysystem.h:800-858); it does not change results.
+        NACT_PARALLEL_THRESH = 32,
         ORDER = Particle::order,
         JP_WORDS = ORDER == 4 ? 14 : ORDER == 6 ? 20 : 26,
         PRED_WORDS = ORDER == 4 ? 7 : ORDER == 6 ? 10 : 13,
@@ -98,6 +104,25 @@ struct Gravity {
     }
 
     void calc_potential(const double eps2, double potbuf[]) { check(hh_potential(ctx, eps2, potbuf), "hh_potential"); }
+
+    // FAST_OMP_SYNC entry points (SRC/nbodysystem.h:767-902; the AVX back
+    // end's are at SRC/hermite6-avx.h:176,496): every thread of the driver's
+    // omp parallel region calls them.  The master thread does the HIP work;
+    // the barrier after the force makes force[] visible to all threads
+    // before their corrector loop.  The driver's own barrier between the
+    // two calls orders the prediction before the force.
+    void predict_all_fast_omp(const double tsys)
+    {
+#pragma omp master
+        predict_all(tsys);
+    }
+
+    void calc_force_on_first_nact_fast_omp(const int nact, const double eps2, Force force[])
+    {
+#pragma omp master
+        calc_force_on_first_nact(nact, eps2, force);
+#pragma omp barrier
+    }
 };
 
 #endif /* HERMITE_HIP_GRAVITY_H */

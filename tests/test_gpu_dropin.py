@@ -3,7 +3,9 @@
 the HIP back end through include/hermite_hip_gravity.h.
 
 With HERMITE_HIP_VARIANT=exact the run reproduces the nact known answers of
-the reference logs and the oracle's energies bit for bit; with fp64 the mean
+the reference logs and the oracle's energies bit for bit, both through the
+serial block loop and through the FAST_OMP_SYNC one (OpenMP threads calling
+predict_all_fast_omp / calc_force_on_first_nact_fast_omp); with fp64 the mean
 nact stays within 1 % of them."""
 import json
 import os
@@ -22,11 +24,15 @@ pytestmark = pytest.mark.gpu
 ORDERS = (4, 6, 8)
 
 
-def run_dropin(order, variant, P, eps2, eta, eta_s, dtmax, tend):
-    exe = os.path.join(O.REF_DIR, f"ref_hip{order}")
+def run_dropin(order, variant, P, eps2, eta, eta_s, dtmax, tend, omp_threads=0):
+    """omp_threads > 0 runs the driver built with -DFAST_OMP_SYNC (its OpenMP
+    block loop calling the *_fast_omp entry points) on that many threads."""
+    exe = os.path.join(O.REF_DIR, f"ref_hip{order}" + ("_omp" if omp_threads else ""))
     if not os.path.exists(exe):
         pytest.skip("oracle/_ref/ref_hipN not built (needs the reference sources)")
     env = dict(os.environ, HERMITE_HIP_VARIANT=variant)
+    if omp_threads:
+        env["OMP_NUM_THREADS"] = str(omp_threads)
     with tempfile.TemporaryDirectory() as td:
         fin = os.path.join(td, "p.bin")
         np.ascontiguousarray(P).tofile(fin)
@@ -37,14 +43,15 @@ def run_dropin(order, variant, P, eps2, eta, eta_s, dtmax, tend):
 
 
 @pytest.mark.parametrize("order", ORDERS)
-def test_reference_driver_on_hip_exact(order):
+@pytest.mark.parametrize("omp_threads", [0, 4])
+def test_reference_driver_on_hip_exact(order, omp_threads):
     eta, eta_s, want1, want2 = NACT_KAT[order]
     n = 1024
     m, x, v = H.plummer(n, 19640916, 100.0)
     P = O.make_particles(order, m, x, v)
     eps2 = (4.0 / n) ** 2
-    print(f"drop-in exact order={order}", flush=True)
-    r = run_dropin(order, "exact", P, eps2, eta, eta_s, 1 / 16, 2.0)
+    print(f"drop-in exact order={order} omp_threads={omp_threads}", flush=True)
+    r = run_dropin(order, "exact", P, eps2, eta, eta_s, 1 / 16, 2.0, omp_threads)
     assert "%e" % (r["num_step_t1"] / r["num_bstep_t1"]) == want1
     assert "%e" % ((r["num_step"] - r["num_step_t1"]) / (r["num_bstep"] - r["num_bstep_t1"])) == want2
     o = O.System(order, P, 0.0, eps2, eta, eta_s, 1 / 16)

@@ -156,7 +156,7 @@ def test_ensemble_fp64_matches_oracle_statistically():
 
 @pytest.mark.parametrize("use_graph", [False, True])
 def test_resident_force_timing_changes_nothing(use_graph):
-    """Event timing around every force launch (event record nodes inside the
+    """Event timing around every force launch (timed runs launch without a
     graph) leaves the integration bit-identical and counts every launch."""
     order, n, eps2 = 6, 500, (1 / 32) ** 2
     o = initialised(order, n, 43, eps2, 0.4, 0.1, 1 / 16)
