@@ -82,6 +82,9 @@ def hip_lib():
         L.hhr_set_timing.argtypes = [vp, C.c_int]
         L.hhr_set_plan.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
         L.hhr_force_timing.argtypes = [vp, _dp, C.POINTER(C.c_longlong)]
+        L.hhr_step_log.argtypes = [vp, C.c_longlong, C.POINTER(C.c_int), C.POINTER(C.c_float),
+                                   C.POINTER(C.c_float)]
+        L.hhr_step_log.restype = C.c_longlong
         _hip = L
     return _hip
 
@@ -427,3 +430,18 @@ class Resident:
         ms, n = C.c_double(), C.c_longlong()
         self._check(hip_lib().hhr_force_timing(self.h, C.byref(ms), C.byref(n)), "hhr_force_timing")
         return ms.value, n.value
+
+    def step_log(self):
+        """(active count, force ms, other ms) arrays of every timed step
+        with active particles since set_timing."""
+        L = hip_lib()
+        n = L.hhr_step_log(self.h, 0, None, None, None)
+        if n < 0:
+            raise HHError(f"hhr_step_log failed ({n})")
+        na = np.zeros(n, dtype=np.int32)
+        fm = np.zeros(n, dtype=np.float32)
+        om = np.zeros(n, dtype=np.float32)
+        if n:
+            L.hhr_step_log(self.h, n, na.ctypes.data_as(C.POINTER(C.c_int)),
+                           fm.ctypes.data_as(C.POINTER(C.c_float)), om.ctypes.data_as(C.POINTER(C.c_float)))
+        return na, fm, om

@@ -641,13 +641,20 @@ struct hhr_ctx {
     int graph_steps = 0;
     double last_run_ms = 0.0;
     long long steps_launched = 0;
-    // Optional per-launch timing of the force kernel: an event pair around
-    // every force launch of a batch.  Timed runs launch their steps directly
-    // on the stream (no graph), so the events are ordinary stream events.
+    // Optional per-step timing: four events per step of a batch (step start,
+    // before and after the force launch, step end).  Timed runs launch their
+    // steps directly on the stream (no graph), so the events are ordinary
+    // stream events.  The plan kernel also writes each step's active count
+    // to steplog, so every timed step is logged as (active count, force ms,
+    // ms of the other three launches).
     bool timing = false;
     std::vector<hipEvent_t> kev;
+    int *steplog = nullptr;
+    int steplog_cap = 0;
     double force_ms = 0.0;
     long long force_launches = 0;
+    std::vector<int> log_nact;
+    std::vector<float> log_force, log_other;
     std::string err;
 };
 
@@ -680,6 +687,8 @@ static RArgs r_args(hhr_ctx *c)
     a.part = c->part;
     a.npad = c->npad;
     a.ntiles_total = c->ntiles;
+    a.steplog = c->timing ? c->steplog : nullptr;
+    a.steplog_cap = c->steplog_cap;
     return a;
 }
 
@@ -693,14 +702,15 @@ template <int ORDER, int MODE>
 static hipError_t r_launch_step(hhr_ctx *c, int s)
 {
     const RArgs a = r_args(c);
+    hipError_t e = r_mark(c, 4 * s);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL((r_compact_kernel<ORDER, MODE == EXACT>), dim3(c->ntiles), dim3(NT), 0, c->stream, a);
     hipLaunchKernelGGL((r_plan_kernel<ORDER>), dim3(1), dim3(NT), 0, c->stream, a);
-    hipError_t e = r_mark(c, 2 * s);
-    if (e != hipSuccess) return e;
+    if ((e = r_mark(c, 4 * s + 1)) != hipSuccess) return e;
     hipLaunchKernelGGL((r_force_kernel<ORDER, MODE>), dim3(c->force_grid), dim3(NT), 0, c->stream, a);
-    e = r_mark(c, 2 * s + 1);
-    if (e != hipSuccess) return e;
+    if ((e = r_mark(c, 4 * s + 2)) != hipSuccess) return e;
     hipLaunchKernelGGL((r_reduce_correct_kernel<ORDER>), dim3(c->force_grid), dim3(NT), 0, c->stream, a);
+    if ((e = r_mark(c, 4 * s + 3)) != hipSuccess) return e;
     return hipGetLastError();
 }
 
@@ -723,6 +733,9 @@ static void r_free_events(hhr_ctx *c)
 {
     for (hipEvent_t e : c->kev) (void)hipEventDestroy(e);
     c->kev.clear();
+    (void)hipFree(c->steplog);
+    c->steplog = nullptr;
+    c->steplog_cap = 0;
 }
 
 static void r_free(hhr_ctx *c)
@@ -920,10 +933,12 @@ int hhr_run(hhr_ctx *c, double tend, int use_graph, int batch_steps)
     RCHK(c, hipMemcpy(hc.data(), c->clu, sizeof(RClu) * c->nclu, hipMemcpyDeviceToHost));
     for (auto &h : hc) h.tend = tend;
     RCHK(c, hipMemcpy(c->clu, hc.data(), sizeof(RClu) * c->nclu, hipMemcpyHostToDevice));
-    if (c->timing && (int)c->kev.size() != 2 * batch_steps) {
+    if (c->timing && (int)c->kev.size() != 4 * batch_steps) {
         r_free_events(c);
-        c->kev.assign(2 * batch_steps, nullptr);
+        c->kev.assign(4 * batch_steps, nullptr);
         for (auto &e : c->kev) RCHK(c, hipEventCreate(&e));
+        RCHK(c, hipMalloc(&c->steplog, sizeof(int) * batch_steps));
+        c->steplog_cap = batch_steps;
     }
     if (use_graph && (!c->gexec || c->graph_steps != batch_steps)) {
         if (c->gexec) RCHK(c, hipGraphExecDestroy(c->gexec));
@@ -956,10 +971,22 @@ int hhr_run(hhr_ctx *c, double tend, int use_graph, int batch_steps)
         RCHK(c, hipMemcpyAsync(&g, c->glob, sizeof(RGlob), hipMemcpyDeviceToHost, c->stream));
         RCHK(c, hipStreamSynchronize(c->stream));
         if (c->timing) {
+            std::vector<int> na(batch_steps);
+            RCHK(c, hipMemcpy(na.data(), c->steplog, sizeof(int) * batch_steps, hipMemcpyDeviceToHost));
             for (int s = 0; s < batch_steps; s++) {
-                float kms = 0.f;
-                RCHK(c, hipEventElapsedTime(&kms, c->kev[2 * s], c->kev[2 * s + 1]));
-                c->force_ms += kms;
+                float t01 = 0.f, t12 = 0.f, t23 = 0.f;
+                RCHK(c, hipEventElapsedTime(&t01, c->kev[4 * s], c->kev[4 * s + 1]));
+                RCHK(c, hipEventElapsedTime(&t12, c->kev[4 * s + 1], c->kev[4 * s + 2]));
+                RCHK(c, hipEventElapsedTime(&t23, c->kev[4 * s + 2], c->kev[4 * s + 3]));
+                c->force_ms += t12;
+                // steps batch_steps before g.step wrote their counts at (step % cap)
+                const long long st = (long long)g.step - batch_steps + s;
+                const int nact = na[(size_t)(st % batch_steps)];
+                if (nact > 0) {
+                    c->log_nact.push_back(nact);
+                    c->log_force.push_back(t12);
+                    c->log_other.push_back(t01 + t23);
+                }
             }
             c->force_launches += batch_steps;
         }
@@ -1018,7 +1045,25 @@ int hhr_set_timing(hhr_ctx *c, int on)
     c->timing = on != 0;
     c->force_ms = 0.0;
     c->force_launches = 0;
+    c->log_nact.clear();
+    c->log_force.clear();
+    c->log_other.clear();
     return HH_OK;
+}
+
+long long hhr_step_log(const hhr_ctx *c, long long cap, int *nact, float *force_ms, float *other_ms)
+{
+    if (!c) return HH_E_ARG;
+    const long long n = (long long)c->log_nact.size();
+    if (cap > 0 && nact && force_ms && other_ms) {
+        const long long m = cap < n ? cap : n;
+        for (long long i = 0; i < m; i++) {
+            nact[i] = c->log_nact[i];
+            force_ms[i] = c->log_force[i];
+            other_ms[i] = c->log_other[i];
+        }
+    }
+    return n;
 }
 
 int hhr_force_timing(const hhr_ctx *c, double *ms_total, long long *launches)

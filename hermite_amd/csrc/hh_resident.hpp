@@ -10,8 +10,8 @@
 //                     that stay inactive
 //   r_plan_kernel     per-cluster dtlim, step counters, kernel choice and
 //                     geometry, task table of the persistent force kernel
-//   r_force_kernel    persistent: workgroups pull (cluster, i-group, j-slice)
-//                     tasks from a device queue and write partial forces
+//   r_force_kernel    persistent: a fixed grid walks the step's table of
+//                     (cluster, i-group, j-slice) tasks, writing partial forces
 //   r_reduce_correct_kernel
 //                     sums each active particle's slices in a fixed order,
 //                     applies the Hermite corrector (SRC/hermite{4,6,8}.h
@@ -69,7 +69,7 @@ struct RClu {
 };
 
 struct RGlob {
-    int next_task, total_tasks, nclu, nactive, exact;
+    int total_tasks, nclu, nactive, exact;
     int error;     // a cluster's block time stopped advancing
     int total_active;
     int step;      // block steps planned; step & 1 selects the current buffers
@@ -87,6 +87,8 @@ struct RArgs {
     double *__restrict__ part;
     int npad;                    // SoA stride = total padded slots
     int ntiles_total;
+    int *__restrict__ steplog;   // optional: active count of step s at [s % steplog_cap]
+    int steplog_cap;
 };
 
 __device__ __forceinline__ double r_inf() { return __longlong_as_double(0x7ff0000000000000LL); }
@@ -264,7 +266,10 @@ template <int ORDER, bool EX>
 __global__ __launch_bounds__(NT) void r_compact_kernel(RArgs a)
 {
     constexpr int W = jp_words(ORDER), Q = pr_words(ORDER);
-    __shared__ double wmin[NT / 64];
+    constexpr int NW = NT / 64;
+    __shared__ double wmin[NW];
+    __shared__ int wcnt[NW];
+    __shared__ int s_base;
     const int tile = blockIdx.x, tid = threadIdx.x;
     const int k = a.tile_clu[tile];
     RClu &c = a.clu[k];
@@ -275,20 +280,25 @@ __global__ __launch_bounds__(NT) void r_compact_kernel(RArgs a)
     const double t = a.tn[i];
     const bool act = live && t == tmin;
     const unsigned long long mask = __ballot(act);
-    const int lane = tid & 63;
-    if (mask != 0) {
-        const int leader = __ffsll((long long)mask) - 1;
-        int base = 0;
-        if (lane == leader) base = atomicAdd(&c.acnt[p], __popcll(mask));
-        base = __shfl(base, leader, 64);
-        if (act) {
-            const unsigned long long below = lane == 0 ? 0ULL : (mask & ((~0ULL) >> (64 - lane)));
-            a.ilist[c.off + base + __popcll(below)] = i;
-        }
-    }
+    const int lane = tid & 63, wave = tid >> 6;
     // minimum block time of the slots this step leaves alone
     double m = r_wave_min(act ? r_inf() : t);
-    if (lane == 0) wmin[tid >> 6] = m;
+    if (lane == 0) {
+        wmin[wave] = m;
+        wcnt[wave] = __popcll(mask);
+    }
+    __syncthreads();
+    // one append per tile (same-address atomics are serialised at the
+    // memory side, so fewer of them is what matters)
+    if (tid == 0) {
+        int tot = 0;
+        for (int w = 0; w < NW; w++) {
+            m = fmin(m, wmin[w]);
+            tot += wcnt[w];
+        }
+        s_base = tot > 0 ? atomicAdd(&c.acnt[p], tot) : 0;
+        if (m < r_inf()) atomicMin(&c.tmin_slot[p ^ 1][tile % R_SLOTS], r_bits(m));
+    }
     if (live) {
         double r[W], q[Q];
 #pragma unroll
@@ -298,9 +308,11 @@ __global__ __launch_bounds__(NT) void r_compact_kernel(RArgs a)
         for (int w = 0; w < Q; w++) a.jpr[(size_t)w * a.npad + i] = q[w];
     }
     __syncthreads();
-    if (tid == 0) {
-        for (int w = 1; w < NT / 64; w++) m = fmin(m, wmin[w]);
-        if (m < r_inf()) atomicMin(&c.tmin_slot[p ^ 1][tile % R_SLOTS], r_bits(m));
+    if (act) {
+        int base = s_base;
+        for (int w = 0; w < wave; w++) base += wcnt[w];
+        const unsigned long long below = lane == 0 ? 0ULL : (mask & ((~0ULL) >> (64 - lane)));
+        a.ilist[c.off + base + __popcll(below)] = i;
     }
 }
 
@@ -320,43 +332,83 @@ __device__ double r_dtlim(double tnext, double dtmax)
 template <int ORDER, bool POT>
 __host__ __device__ constexpr int r_nc() { return n_acc(ORDER, POT); }
 
-// One workgroup.  Geometry per cluster from its active count; the task
-// budget is shared in proportion to each cluster's nact * n work.
+// Exclusive scan over the NT threads of a workgroup, in thread order
+// (integer sums, so the grouping does not matter); returns this thread's
+// prefix and leaves the total in *total.
+template <typename T>
+__device__ T r_block_exscan(T v, T *s_buf, T *total)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    T x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const T y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) s_buf[wave] = x;
+    __syncthreads();
+    T before = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; w++) {
+        if (w < wave) before += s_buf[w];
+        all += s_buf[w];
+    }
+    __syncthreads();
+    *total = all;
+    return before + x - v;
+}
+
+// One workgroup; thread t owns the clusters [t*cpt, (t+1)*cpt).  Geometry
+// per cluster from its active count; the task budget is shared in
+// proportion to each cluster's nact * n work; task, active-list and partial
+// offsets are block scans in cluster order.
 template <int ORDER>
 __global__ __launch_bounds__(NT) void r_plan_kernel(RArgs a)
 {
     constexpr int NC = f_words(ORDER);
-    __shared__ double s_work[NT];
-    __shared__ double s_total;
-    const int tid = threadIdx.x;
+    constexpr int NW = NT / 64;
+    __shared__ double s_wsum[NW];
+    __shared__ long long s_scan[NW];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nclu = a.glob->nclu;
     const int p = a.glob->step & 1;
+    const bool exact = a.glob->exact != 0;
+    const RPlan pl = a.glob->plan;
+    const int cpt = (nclu + NT - 1) / NT;
+    const int k0 = min(tid * cpt, nclu), k1 = min(k0 + cpt, nclu);
+
+    // active counts and block-time minima of the owned clusters; both
+    // parity buffers of this step are cleared for their next use
     double w = 0.0;
-    for (int k = tid; k < nclu; k += NT) {
+    for (int k = k0; k < k1; k++) {
         RClu &c = a.clu[k];
         c.nact = c.acnt[p];
         c.acnt[p] = 0;
         c.mode = 0;
         c.ntask = 0;
+        double tmin = r_inf();
+#pragma unroll
+        for (int q = 0; q < R_SLOTS; q++) tmin = fmin(tmin, __longlong_as_double((long long)c.tmin_slot[p][q]));
+#pragma unroll
+        for (int q = 0; q < R_SLOTS; q++) c.tmin_slot[p][q] = r_bits(r_inf());
+        c.tnext = tmin;
         if (c.nact > 0) w += (double)c.nact * c.n;
     }
-    s_work[tid] = w;
+    // total work: per-wave xor tree, then the waves in order (fixed order)
+    w = wave_sum(w);
+    if (lane == 0) s_wsum[wave] = w;
     __syncthreads();
-    if (tid == 0) {
-        double t = 0.0;
-        for (int q = 0; q < NT; q++) t += s_work[q];
-        s_total = t;
-    }
-    __syncthreads();
-    const double total = s_total;
-    for (int k = tid; k < nclu; k += NT) {
+    double total = 0.0;
+#pragma unroll
+    for (int q = 0; q < NW; q++) total += s_wsum[q];
+
+    const int gs = NT * r_ni(exact ? EXACT : FP64);
+    int my_tasks = 0, my_act = 0, my_live = 0;
+    long long my_part = 0;
+    for (int k = k0; k < k1; k++) {
         RClu &c = a.clu[k];
-        double tmin = r_inf();
-        for (int q = 0; q < R_SLOTS; q++) {
-            tmin = fmin(tmin, __longlong_as_double((long long)c.tmin_slot[p][q]));
-            c.tmin_slot[p][q] = r_bits(r_inf());
-        }
         if (c.nact == 0) continue;
+        const double tmin = c.tnext;
         const double dtlim = r_dtlim(tmin, c.dtmax);
         if (!(tmin > c.tsys) || dtlim <= 0.0) {  // collapsed step: stop this cluster
             c.nact = 0;
@@ -364,54 +416,77 @@ __global__ __launch_bounds__(NT) void r_plan_kernel(RArgs a)
             atomicExch(&a.glob->error, 1);
             continue;
         }
-        c.tnext = tmin;
         c.dtlim = dtlim;
         c.num_step += c.nact;
         c.num_bstep += 1;
         c.tsys = tmin;
-        const RPlan &pl = a.glob->plan;
         const double share = ((double)c.nact * c.n) / total;
-        if (a.glob->exact || c.nact > pl.jpar_max) {
+        if (exact || c.nact > pl.jpar_max) {
             const int budget = max(1, (int)ceil(pl.task_target * share));
             c.mode = 1;
-            c.ngrp = (c.nact + NT * r_ni(a.glob->exact ? EXACT : FP64) - 1) / (NT * r_ni(a.glob->exact ? EXACT : FP64));
-            int ns = a.glob->exact ? 1 : min(min(c.ntiles, pl.smax), max(1, (budget + c.ngrp - 1) / c.ngrp));
+            c.ngrp = (c.nact + gs - 1) / gs;
+            const int ns = exact ? 1 : min(min(c.ntiles, pl.smax), max(1, (budget + c.ngrp - 1) / c.ngrp));
             c.tps = (c.ntiles + ns - 1) / ns;
             c.nslice = (c.ntiles + c.tps - 1) / c.tps;
+            c.ipad = c.ngrp * gs;
         } else {
             const int budget = max(1, (int)ceil(pl.jpar_target * share));
             c.mode = 2;
             c.ngrp = (c.nact + R_GI - 1) / R_GI;
-            int ns = min(c.ntiles, max(1, (budget + c.ngrp - 1) / c.ngrp));
+            const int ns = min(c.ntiles, max(1, (budget + c.ngrp - 1) / c.ngrp));
             const int tpc = (c.ntiles + ns - 1) / ns;  // tiles per chunk
             c.jchunk = tpc * TJ;
             c.nslice = (c.ntiles + tpc - 1) / tpc;
+            c.ipad = c.ngrp * R_GI;
         }
         c.ntask = c.ngrp * c.nslice;
+        my_tasks += c.ntask;
+        my_act += c.nact;
+        my_live += 1;
+        my_part += (long long)c.nslice * NC * c.ipad;
     }
-    __syncthreads();
-    if (tid == 0) {
-        int t0 = 0, act = 0, nactive = 0;
-        long long p0 = 0;
-        for (int k = 0; k < nclu; k++) {
-            RClu &c = a.clu[k];
-            c.task0 = t0;
-            c.act0 = act;
-            c.pofs = p0;
-            t0 += c.ntask;
-            if (c.mode != 0) {
-                nactive++;
-                act += c.nact;
-                c.ipad = c.ngrp * (c.mode == 1 ? NT * r_ni(a.glob->exact ? EXACT : FP64) : R_GI);
-                p0 += (long long)c.nslice * NC * c.ipad;
-            }
+    // offsets in cluster order
+    long long tot_tasks, tot_act, tot_live, tot_part;
+    long long t0 = r_block_exscan<long long>(my_tasks, s_scan, &tot_tasks);
+    long long act = r_block_exscan<long long>(my_act, s_scan, &tot_act);
+    (void)r_block_exscan<long long>(my_live, s_scan, &tot_live);
+    long long p0 = r_block_exscan<long long>(my_part, s_scan, &tot_part);
+    for (int k = k0; k < k1; k++) {
+        RClu &c = a.clu[k];
+        c.task0 = (int)t0;
+        c.act0 = (int)act;
+        c.pofs = p0;
+        t0 += c.ntask;
+        if (c.mode != 0) {
+            act += c.nact;
+            p0 += (long long)c.nslice * NC * c.ipad;
         }
-        a.glob->total_tasks = t0;
-        a.glob->next_task = 0;
-        a.glob->nactive = nactive;
-        a.glob->total_active = act;
+    }
+    if (tid == 0) {
+        a.glob->total_tasks = (int)tot_tasks;
+        a.glob->nactive = (int)tot_live;
+        a.glob->total_active = (int)tot_act;
+        if (a.steplog) a.steplog[a.glob->step % a.steplog_cap] = (int)tot_act;
         a.glob->step += 1;
     }
+}
+
+// Clusters whose first task (first active-list position) the force (reduce)
+// kernel keeps in LDS for its lookups; larger ensembles search the cluster
+// records in global memory.
+constexpr int R_CLU_LDS = 2048;
+
+// The last cluster whose key (task0 or act0) is <= x, from an LDS copy of
+// the keys (keys[k], k < nclu) or from the records themselves.
+__device__ __forceinline__ int r_find_lds(const int *keys, int nclu, int x)
+{
+    int lo = 0, hi = nclu - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (keys[mid] <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
 }
 
 // Cluster of a task: the last cluster whose first task is <= task.
@@ -426,11 +501,10 @@ __device__ int r_find_cluster(const RClu *clu, int nclu, int task)
     return lo;
 }
 
-// Persistent force kernel: workgroups pull tasks of either kind from one
-// queue.  (Measured: one kernel with both task kinds runs the tile tasks
-// faster than a tile-only kernel with fewer registers and twice the waves
-// per SIMD; the j-parallel path's register budget leaves the tile loop two
-// waves per SIMD, which suits its LDS traffic.)
+// Persistent force kernel: a fixed grid of workgroups walks the step's task
+// table, which holds tasks of either kind.  (Measured: one kernel with both
+// task kinds runs the tile tasks faster than a tile-only kernel with fewer
+// registers and twice the waves per SIMD.)
 template <int ORDER, int MODE>
 __global__ __launch_bounds__(NT) void r_force_kernel(RArgs a)
 {
@@ -443,20 +517,25 @@ __global__ __launch_bounds__(NT) void r_force_kernel(RArgs a)
     constexpr int FOLD = 8;
     __shared__ double sj[TJ * L];
     __shared__ double red[NW][V];
-    __shared__ int s_task;
+    __shared__ int s_key[R_CLU_LDS];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int npad = a.npad;
     const int nclu = a.glob->nclu;
     const int total = a.glob->total_tasks;
+    const bool keys_lds = nclu > 1 && nclu <= R_CLU_LDS;
+    if (keys_lds) {
+        for (int k = tid; k < nclu; k += NT) s_key[k] = a.clu[k].task0;
+        __syncthreads();
+    }
 
-    for (;;) {
-        __syncthreads();
-        if (tid == 0) s_task = atomicAdd(&a.glob->next_task, 1);
-        __syncthreads();
-        const int task = s_task;
-        if (task >= total) break;
-        const int k = r_find_cluster(a.clu, nclu, task);
+    // Static round-robin over the task table: no queue counter (a shared
+    // counter is a same-address atomic per task).  Tasks of one step have
+    // near-equal work (the plan shares the budget by nact * n), and with a
+    // grid that is a multiple of 8 task t runs on XCD t % 8, so slices that
+    // are a multiple of 8 apart share an L2.
+    for (int task = blockIdx.x; task < total; task += gridDim.x) {
+        const int k = nclu == 1 ? 0 : keys_lds ? r_find_lds(s_key, nclu, task) : r_find_cluster(a.clu, nclu, task);
         const RClu &c = a.clu[k];
         const int local = task - c.task0;
         const int grp = local / c.nslice, slice = local - grp * c.nslice;
@@ -624,7 +703,14 @@ __global__ __launch_bounds__(NT) void r_reduce_correct_kernel(RArgs a)
 {
     constexpr int NC = f_words(ORDER);
     constexpr int PB = NT / RL;  // particles per workgroup pass
+    __shared__ int s_key[R_CLU_LDS];
     const int nclu = a.glob->nclu, total = a.glob->total_active;
+    if (blockIdx.x * PB >= total) return;  // whole workgroup idle this step
+    const bool keys_lds = nclu > 1 && nclu <= R_CLU_LDS;
+    if (keys_lds) {
+        for (int k = threadIdx.x; k < nclu; k += NT) s_key[k] = a.clu[k].act0;
+        __syncthreads();
+    }
     const int l = threadIdx.x % RL;
     const int nb = a.glob->step & 1;  // the plan has advanced the step: next buffers
     // each lane keeps the minimum of the new block times it produced for
@@ -634,7 +720,7 @@ __global__ __launch_bounds__(NT) void r_reduce_correct_kernel(RArgs a)
     for (int base = blockIdx.x * PB; base < total; base += gridDim.x * PB) {
         const int pos = base + threadIdx.x / RL;
         const bool live = pos < total;
-        const int k = live ? r_find_active(a.clu, nclu, pos) : 0;
+        const int k = !live || nclu == 1 ? 0 : keys_lds ? r_find_lds(s_key, nclu, pos) : r_find_active(a.clu, nclu, pos);
         const RClu &c = a.clu[k];
         const int ii = pos - c.act0;
         double S[NC];
@@ -642,6 +728,7 @@ __global__ __launch_bounds__(NT) void r_reduce_correct_kernel(RArgs a)
         for (int q = 0; q < NC; q++) S[q] = 0.0;
         if (live) {
             const double *part = a.part + c.pofs;
+#pragma unroll 4
             for (int sl = l; sl < c.nslice; sl += RL) {
 #pragma unroll
                 for (int q = 0; q < NC; q++) S[q] += part[((size_t)sl * NC + q) * c.ipad + ii];
