@@ -1,28 +1,32 @@
 """Benchmark of the Hermite force + predictor hot path on one MI355X.
 
-Workload: 6th-order Hermite, Plummer sphere N = 65536 from seed 2 (Aarseth-
-Henon-Wielen sampler, radii truncated at 22.8, Henon units, equal masses),
-eps = 1/1024, eta = 0.2 (eta_s = 0.1 for the first steps), power-of-two
-block timesteps with dt_max = 1/16.  init_step (SRC/nbodysystem.h:565-631)
-runs once on the GPU through the host driver; the block-step integration
-then runs GPU-resident (schedule, predict_all, force on the active list,
-corrector and timestep on the device).  The timed steps are launched
-directly on one stream with a HIP event pair around every force launch;
---graph replays them from a hipGraph instead, without the events.
+Default workload (c2): 6th-order Hermite, Plummer sphere N = 65536 from
+seed 2 (Aarseth-Henon-Wielen sampler, radii truncated at 22.8, Henon units,
+equal masses), eps = 1/1024, eta = 0.2 (eta_s = 0.1 for the first steps),
+power-of-two block timesteps with dt_max = 1/16.  --workload selects one of
+the other configurations (hermite_amd/workloads.py): c1 (4th order,
+N = 1024), c3 (8th order, N = 2^18, t = 0 -> 1/8), c4 (256 clusters of
+256..8192 particles, 4th order) or c5 (6th order, N = 2^20 with Salpeter
+masses, dt down to 2^-40, t = 0 -> 1/16).
 
-One step is one dt_max interval of integration (the reference's
+init_step (SRC/nbodysystem.h:565-631) runs on the GPU through the host
+driver; the block-step integration then runs GPU-resident (schedule,
+predict_all, force on the active list, corrector and timestep on the
+device).  One step is one dt_max interval of integration (the reference's
 integrate_one_dtmax, SRC/nbodysystem.h:904-944): the timed region is
-steps x 1/16 time units from t = 0 (default 16 steps: t = 0 -> 1), started
-and ended by a stream synchronisation inside hhr_run.  Warmup steps run on a
+steps x dt_max time units from t = 0, started and ended by a stream
+synchronisation inside hhr_run, with the block steps launched directly on
+one stream (--graph: replayed from a hipGraph).  Warmup intervals run on a
 separate context that is discarded, so the timed run starts from t = 0.
 
-value = interactions/s of the plain FP64 kernel (num_step * N / wall), with
-|dE/E| at the end of the interval.  Beside it: the split-precision kernel's
-integration outcome, both kernels' all-pairs force rate (the reference's
-"Init N square" evaluation), the split kernel's force error against the FP64
-kernel, the force kernel's roofline (HIP events around every force launch of
-the timed run) and the reference's own AVX2 back end and driver, built from
-its sources, timed on the host cores (cpu_baseline).
+value = interactions/s of the plain FP64 kernel over the timed run
+(sum of nact * N over block steps / wall), with |dE/E| at its end.  A second,
+event-timed pass of the same integration gives the force kernel's mean
+launch time for the roofline and the time split by active count.  Beside
+it: the split-precision kernel's integration outcome, both kernels'
+all-pairs force rate (the reference's "Init N square" evaluation), the split
+kernel's force error against the FP64 kernel, and the reference's own AVX2
+back end, built from its sources, timed on the host cores (cpu_baseline).
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -40,12 +44,15 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 import hermite_amd as H  # noqa: E402
+from hermite_amd import workloads as W  # noqa: E402
 
 METRIC = "interactions/s and Gflops (60/97/144 flop conv.), split-prec vs FP64 at stated |ΔE/E|"
-ORDER, N, SEED, EPS, ETA, ETA_S, DTMAX = 6, 65536, 2, 1.0 / 1024, 0.2, 0.1, 1.0 / 16
-PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector peak (also the unpacked FP32 issue rate)
-BATCH = 16               # block steps per graph launch
+PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector peak
+BATCH = 16               # block steps per launch batch
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "r_force_kernel_6th_n65536_traffic.txt")
+
+# Kept for the tests and tools that import the default workload's constants.
+ORDER, N, SEED, EPS, ETA, ETA_S, DTMAX = 6, 65536, 2, 1.0 / 1024, 0.2, 0.1, 1.0 / 16
 
 
 def log(msg):
@@ -53,7 +60,8 @@ def log(msg):
 
 
 def initial_state(eps2):
-    """Particle records after init_step on the GPU (FP64 kernel)."""
+    """Particle records of the default workload after init_step on the GPU
+    (FP64 kernel), its initial energy, and the raw records."""
     m, x, v = H.plummer(N, SEED)
     P = H.make_particles(ORDER, m, x, v)
     s = H.System(ORDER, P, 0.0, eps2, ETA, ETA_S, DTMAX, variant="fp64")
@@ -64,52 +72,75 @@ def initial_state(eps2):
         s.close()
 
 
-def integrate(variant, P0, e0, eps2, tend, timing, graph=False):
-    r = H.Resident(ORDER, variant, [(P0, eps2, ETA, DTMAX, 0.0, 0.0)])
+def initial_states(w):
+    """(records after init_step, eps2, E0, raw records) per cluster of w."""
+    out = []
+    for c in w.clusters:
+        P = W.cluster_state(w.order, c)
+        s = H.System(w.order, P, 0.0, c.eps ** 2, w.eta, w.eta_s, w.dtmax, w.dtmin, variant="fp64")
+        try:
+            s.init_step()
+            out.append((s.ptcl(), c.eps ** 2, s.init_energy, P))
+        finally:
+            s.close()
+    return out
+
+
+def integrate(w, variant, states, tend, timing=False, graph=False):
+    clus = [(P0, eps2, w.eta, w.dtmax, w.dtmin, 0.0) for P0, eps2, _, _ in states]
+    r = H.Resident(w.order, variant, clus)
     try:
         if timing:
             r.set_timing(True)
         t0 = time.perf_counter()
         err = None
         try:
-            r.run(tend, use_graph=graph, batch=BATCH)
+            r.run(tend, use_graph=graph and not timing, batch=BATCH)
         except H.HHError as ex:
             err = str(ex)
         wall = time.perf_counter() - t0
-        st = r.stats()
-        out = {"wall_s": wall, "num_step": st["num_step"], "num_bstep": st["num_bstep"], "tsys": st["tsys"],
-               "error": err}
+        st = [r.stats(k) for k in range(len(states))]
+        inter = float(sum(s["num_step"] * len(states[k][0]) for k, s in enumerate(st)))
+        out = {"wall_s": wall, "interactions": inter, "num_step": sum(s["num_step"] for s in st),
+               "num_bstep": max(s["num_bstep"] for s in st), "tsys": min(s["tsys"] for s in st),
+               "dtmin_hits": sum(s["dtmin_hits"] for s in st), "error": err}
         if err is None:
-            out["dE_E"] = (r.energy() - e0) / e0
+            de = np.array([abs((r.energy(k) - states[k][2]) / states[k][2]) for k in range(len(states))])
+            out["dE_E"] = float(de.max())
+            if len(de) > 1:
+                out["dE_E_median"] = float(np.median(de))
         if timing:
             out["force_ms"], out["force_launches"] = r.force_timing()
-        out["steps_launched"] = r.steps_launched
+            out["log"] = r.step_log()
         return out
     finally:
         r.close()
 
 
-def all_pairs(P0, eps2):
+def all_pairs(order, P0, eps2):
     """Force on all N from all N (inputs resident) for both kernels, and the
     split kernel's per-particle relative error against the FP64 kernel."""
+    n = len(P0)
     res, F = {}, {}
     for variant in ("fp64", "split"):
-        g = H.Gravity(ORDER, N, variant)
+        g = H.Gravity(order, n, variant)
         try:
             g.set_jp_all(P0)
             g.predict_all(0.0)
-            g.time_force(N, eps2, 2)
-            ms = g.time_force(N, eps2, 20)
-            F[variant] = g.calc_force_on_first_nact(N, eps2)
+            ms0 = g.time_force(n, eps2, 1)
+            reps = int(max(1, min(20, 3000.0 / max(ms0, 1e-3))))
+            ms = g.time_force(n, eps2, reps)
+            F[variant] = g.calc_force_on_first_nact(n, eps2)
             geo = g.last_geometry()
         finally:
             g.close()
-        inter = float(N) * N
+        inter = float(n) * n
         res[variant] = {"force_ms": ms, "interactions_per_s": inter / (ms * 1e-3),
-                        "gflops": inter * H.FLOPS[ORDER] / (ms * 1e-3) * 1e-9, "geometry": geo}
+                        "gflops": inter * H.FLOPS[order] / (ms * 1e-3) * 1e-9, "geometry": geo}
         log(f"all-pairs {variant}: {ms:.3f} ms  {res[variant]['interactions_per_s']:.3e} int/s")
     err = {}
-    for k, name in enumerate(("acc", "jrk", "snp")):
+    names = ("acc", "jrk", "snp", "crk")[:H.FSTRIDE[order] // 3]
+    for k, name in enumerate(names):
         d = np.linalg.norm(F["split"][:, 3 * k:3 * k + 3] - F["fp64"][:, 3 * k:3 * k + 3], axis=1)
         ref = np.linalg.norm(F["fp64"][:, 3 * k:3 * k + 3], axis=1)
         rel = d / ref
@@ -119,9 +150,12 @@ def all_pairs(P0, eps2):
     return res
 
 
-def traffic_from_profile():
+def traffic_from_profile(w):
     """HBM bytes per force launch measured with rocprofv3 --pmc (FETCH_SIZE +
-    WRITE_SIZE, calibrated against a copy of known size), if recorded."""
+    WRITE_SIZE, calibrated against a copy of known size), if recorded for
+    this workload."""
+    if w.name != "c2":
+        return None
     try:
         with open(TRAFFIC_FILE) as f:
             for ln in f:
@@ -132,17 +166,7 @@ def traffic_from_profile():
     return None
 
 
-def cpu_baseline(P_raw, eps2, tend):
-    """The reference's own driver and AVX2 back end (SRC/nbodysystem.h with
-    SRC/hermite6-avx.h, built from its sources with its own flags into
-    oracle/_ref) integrating the same initial state from t = 0."""
-    try:
-        import oracle
-        ref = oracle.RefAVX(ORDER)
-    except (ImportError, FileNotFoundError):
-        return None
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
-    info = ref.run(P_raw, eps2, ETA, ETA_S, DTMAX, tend, threads=threads)
+def _cpu_model():
     cpu = platform.processor() or platform.machine()
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
@@ -151,29 +175,96 @@ def cpu_baseline(P_raw, eps2, tend):
                 cpu = ln.split(":", 1)[1].strip()
     except OSError:
         pass
-    inter = float(info["num_step"]) * len(P_raw)
-    return {
-        "value": inter / info["seconds"],
-        "unit": "interactions/s",
-        "cores": info["threads"],
-        "kind": "reference",
-        "sample": f"reference driver + AVX2 back end, same N=65536 6th-order state, t=0->{tend:g} "
-                  f"({info['num_bstep']} block steps, {info['num_step']} particle steps, "
-                  f"{info['seconds']:.1f} s, dE/E {info['de']:.2e}), {cpu}",
-        "gflops": inter / info["seconds"] * H.FLOPS[ORDER] * 1e-9,
-    }
+    return cpu
+
+
+def cpu_baseline(w, states, cpu_tend, budget_s=20.0):
+    """The reference's own driver and AVX2 back end (SRC/nbodysystem.h with
+    SRC/hermite{4,6,8}-avx.h, built from its sources with its own flags into
+    oracle/_ref) on the same initial states.  Single systems of up to 2^16
+    particles integrate t = 0 -> cpu_tend; larger ones time all-pairs force
+    calls on 4096 active particles; the ensemble integrates its clusters in
+    order until about budget_s seconds of CPU work."""
+    try:
+        import oracle
+        ref = oracle.RefAVX(w.order)
+    except (ImportError, FileNotFoundError):
+        return None
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+    cpu = _cpu_model()
+    if len(states) == 1 and len(states[0][3]) <= 65536:
+        P_raw, eps2 = states[0][3], states[0][1]
+        info = ref.run(P_raw, eps2, w.eta, w.eta_s, w.dtmax, cpu_tend, threads=threads)
+        inter = float(info["num_step"]) * len(P_raw)
+        sample = (f"reference driver + AVX2 back end, same N={len(P_raw)} state, t=0->{cpu_tend:g} "
+                  f"({info['num_bstep']} block steps, {info['num_step']} particle steps, {info['seconds']:.1f} s, "
+                  f"dE/E {info['de']:.2e}), {cpu}")
+        secs, used = info["seconds"], info["threads"]
+    elif len(states) == 1:
+        P0, eps2 = states[0][0], states[0][1]
+        nact = 4096
+        _, info = ref.forces(P0, 0.0, nact, eps2, reps=2, threads=threads, want_output=False)
+        inter = 2.0 * nact * len(P0)
+        secs, used = info["seconds"], info["threads"]
+        sample = (f"reference AVX2 back end, force on {nact} active particles from all N={len(P0)} of the same "
+                  f"state, 2 calls ({secs:.1f} s), {cpu}")
+    else:
+        inter, secs, used, done = 0.0, 0.0, threads, 0
+        for P0, eps2, _, P_raw in states:
+            if len(P_raw) % 4:  # the AVX back end takes i in groups of 4 (SRC/hermite4-avx.h:289)
+                continue
+            info = ref.run(P_raw, eps2, w.eta, w.eta_s, w.dtmax, cpu_tend, threads=threads)
+            inter += float(info["num_step"]) * len(P_raw)
+            secs += info["seconds"]
+            used = info["threads"]
+            done += 1
+            if secs > budget_s:
+                break
+        sample = (f"reference driver + AVX2 back end on the first {done} clusters of the ensemble whose N_k is "
+                  f"a multiple of 4, each "
+                  f"t=0->{cpu_tend:g} ({secs:.1f} s), {cpu}")
+    return {"value": inter / secs, "unit": "interactions/s", "cores": used, "kind": "reference",
+            "sample": sample, "gflops": inter / secs * H.FLOPS[w.order] * 1e-9}
+
+
+def by_active_count(log_, n_j):
+    """Force-kernel time split by the step's active count (events)."""
+    na, fm, om = log_
+    out = []
+    for lo, hi in ((1, 64), (65, 1024), (1025, 16384), (16385, 1 << 40)):
+        m = (na >= lo) & (na <= hi)
+        if m.any():
+            out.append({"nact": f"{lo}-{hi}" if hi < (1 << 40) else f">={lo}", "steps": int(m.sum()),
+                        "force_ms": round(float(fm[m].sum()), 2), "other_ms": round(float(om[m].sum()), 2),
+                        "interaction_share": round(float(na[m].sum() / max(na.sum(), 1)), 4)})
+    return out
+
+
+def workload_text(w, tend, graph):
+    kind = "hipGraph replay" if graph else "direct launches on one stream"
+    if w.name == "c4":
+        return (f"4th-order Hermite ensemble of {len(w.clusters)} Plummer clusters, N_k 256..8192 "
+                f"(sum {w.n_total}), eps=4/N_k, eta=0.1, dt_max=1/16, t=0->{tend:g}, plain FP64 kernel, "
+                f"all clusters in shared GPU-resident launches ({kind})")
+    c = w.clusters[0]
+    mass = "Salpeter masses" if c.mass == "salpeter" else "equal masses"
+    return (f"{w.order}th-order Hermite block-step integration, Plummer N={c.n} seed {c.seed} ({mass}), "
+            f"eps=1/{round(1 / c.eps)}, eta={w.eta:g}, dt_max=1/{round(1 / w.dtmax)}"
+            + (f", dt_min=2^{round(np.log2(w.dtmin))}" if w.dtmin > 0 else "")
+            + f", t=0->{tend:g}, plain FP64 kernel, GPU-resident steps ({kind})")
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=16, help="dt_max intervals timed (16 = t 0 -> 1)")
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--steps", type=int, default=0,
+                    help="dt_max intervals timed (default: the workload's whole span, 16 for c2 = t 0 -> 1)")
     ap.add_argument("--warmup", type=int, default=1, help="dt_max intervals on a discarded context")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-tend", type=float, default=DTMAX, help="time span of the CPU baseline run")
-    ap.add_argument("--graph", action="store_true",
-                    help="replay the timed steps from a hipGraph (no per-launch events: roofline from a second, "
-                         "event-timed pass)")
+    ap.add_argument("--cpu-tend", type=float, default=0.0, help="time span of the CPU baseline run (default dt_max)")
+    ap.add_argument("--graph", action="store_true", help="replay the timed steps from a hipGraph")
+    ap.add_argument("--no-split", action="store_true", help="skip the split-precision legs")
     ap.add_argument("--calibrate", action="store_true",
                     help="first run 5 device-to-device copies of 256 MiB (the reference point that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
@@ -192,73 +283,81 @@ def main():
         torch.cuda.synchronize()
         log("calibration copies done (5 x 256 MiB device-to-device)")
         del src, dst
-    eps2 = EPS * EPS
-    tend = args.steps * DTMAX
-    log("init_step on the GPU")
-    P0, e0, P_raw = initial_state(eps2)
+    w = W.workload(args.workload)
+    steps = args.steps or int(round(w.tend / w.dtmax))
+    tend = steps * w.dtmax
+    cpu_tend = args.cpu_tend or w.dtmax
+    log(f"workload {w.name}: init_step on the GPU ({len(w.clusters)} cluster(s), N = {w.n_total})")
+    states = initial_states(w)
     if args.warmup > 0:
         log(f"warmup: {args.warmup} interval(s) on a discarded context")
-        integrate("fp64", P0, e0, eps2, args.warmup * DTMAX, timing=not args.graph, graph=args.graph)
+        integrate(w, "fp64", states, min(args.warmup, steps) * w.dtmax, graph=args.graph)
     log(f"timed: FP64 resident integration t=0->{tend:g}")
-    f = integrate("fp64", P0, e0, eps2, tend, timing=not args.graph, graph=args.graph)
+    f = integrate(w, "fp64", states, tend, graph=args.graph)
     if f["error"]:
         raise SystemExit(f"FP64 integration failed: {f['error']}")
-    if args.graph:
-        log("event-timed pass for the roofline")
-        ft = integrate("fp64", P0, e0, eps2, tend, timing=True)
-        f["force_ms"], f["force_launches"] = ft["force_ms"], ft["force_launches"]
-    inter = float(f["num_step"]) * N
-    value = inter / f["wall_s"]
+    value = f["interactions"] / f["wall_s"]
     log(f"FP64: {f['wall_s']:.3f} s, {f['num_bstep']} block steps, {value:.3e} int/s, dE/E {f['dE_E']:.3e}")
-    log("split-precision integration, one interval")
-    s = integrate("split", P0, e0, eps2, DTMAX, timing=False)
-    log(f"split: stopped at t={s['tsys']:.3e} after {s['num_bstep']} block steps: {s['error']}")
-    ap2 = all_pairs(P0, eps2)
-    # roofline of the dominant kernel (the persistent force kernel)
-    launches = max(f["force_launches"], 1)
-    avg_ms = f["force_ms"] / launches
-    flops_per_launch = inter * H.FLOPS[ORDER] / launches
+    log("event-timed pass of the same integration (roofline)")
+    ft = integrate(w, "fp64", states, tend, timing=True)
+    launches = max(ft["force_launches"], 1)
+    avg_ms = ft["force_ms"] / launches
+    flops_per_launch = ft["interactions"] * H.FLOPS[w.order] / launches
     achieved = flops_per_launch / (avg_ms * 1e-3) * 1e-12
+    split, ap2 = None, None
+    if not args.no_split:
+        split_tend = tend if w.order == 4 else w.dtmax
+        log(f"split-precision integration t=0->{split_tend:g}")
+        s = integrate(w, "split", states, split_tend)
+        log(f"split: t={s['tsys']:.3e} after {s['num_bstep']} block steps: {s['error'] or 'ok'}")
+        split = {"integration_error": s["error"], "tsys_reached": s["tsys"], "block_steps": s["num_bstep"],
+                 "interactions_per_s": s["interactions"] / s["wall_s"], "dE_E": s.get("dE_E"),
+                 "dE_E_median": s.get("dE_E_median")}
+        if len(states) == 1:
+            ap2 = all_pairs(w.order, states[0][0], states[0][1])
+            split.update({"all_pairs_interactions_per_s": ap2["split"]["interactions_per_s"],
+                          "all_pairs_over_fp64": ap2["split_over_fp64"],
+                          "rel_error_vs_fp64": ap2["split_rel_error_vs_fp64"]})
     cpu = None
     if not args.no_cpu:
-        log(f"CPU baseline: reference AVX2 driver t=0->{args.cpu_tend:g}")
-        cpu = cpu_baseline(P_raw, eps2, args.cpu_tend)
+        log(f"CPU baseline: reference AVX2 back end (t=0->{cpu_tend:g} where it integrates)")
+        cpu = cpu_baseline(w, states, cpu_tend)
+    n_j = w.n_total
     line = {
         "metric": METRIC,
         "value": value,
         "unit": "interactions/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * f["wall_s"] / args.steps,
+        "ms_per_step": 1e3 * f["wall_s"] / steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded Plummer sphere)",
-        "config": {"workload": f"6th-order Hermite block-step integration, Plummer N=65536 seed 2, eps=1/1024, "
-                               f"eta=0.2, dt_max=1/16, t=0->{tend:g}, plain FP64 kernel, GPU-resident steps "
-                               + ("replayed from a hipGraph" if args.graph else "launched on one stream"),
-                   "order": ORDER, "n": N, "kernel": "fp64"},
-        "gflops": value * H.FLOPS[ORDER] * 1e-9,
+        "data": "synthetic (seeded Plummer spheres)",
+        "config": {"workload": workload_text(w, tend, args.graph), "name": w.name, "order": w.order,
+                   "n": n_j, "clusters": len(w.clusters), "kernel": "fp64"},
+        "gflops": value * H.FLOPS[w.order] * 1e-9,
         "dE_E": f["dE_E"],
         "block_steps": f["num_bstep"],
-        "mean_nact": f["num_step"] / f["num_bstep"],
-        "us_per_block_step": 1e6 * f["wall_s"] / f["num_bstep"],
+        "mean_nact": f["num_step"] / max(f["num_bstep"], 1),
+        "us_per_block_step": 1e6 * f["wall_s"] / max(f["num_bstep"], 1),
+        "dtmin_hits": f["dtmin_hits"],
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic_from_profile(),
-                     "kernel": "r_force_kernel<6,FP64>", "flops_per_interaction": H.FLOPS[ORDER],
-                     "avg_launch_ms": avg_ms, "launches": f["force_launches"],
-                     "force_share_of_wall": f["force_ms"] * 1e-3 / f["wall_s"]},
-        "split": {"integration_error": s["error"], "tsys_reached": s["tsys"], "block_steps": s["num_bstep"],
-                  "all_pairs_interactions_per_s": ap2["split"]["interactions_per_s"],
-                  "all_pairs_over_fp64": ap2["split_over_fp64"],
-                  "rel_error_vs_fp64": ap2["split_rel_error_vs_fp64"]},
-        "all_pairs_fp64": {"interactions_per_s": ap2["fp64"]["interactions_per_s"], "gflops": ap2["fp64"]["gflops"],
-                           "force_ms": ap2["fp64"]["force_ms"],
-                           "frac": ap2["fp64"]["gflops"] * 1e-3 / PEAK_FP64_TFLOPS},
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic_from_profile(w),
+                     "kernel": f"r_force_kernel<{w.order},FP64>", "flops_per_interaction": H.FLOPS[w.order],
+                     "avg_launch_ms": avg_ms, "launches": ft["force_launches"],
+                     "force_share_of_timed_wall": ft["force_ms"] * 1e-3 / ft["wall_s"],
+                     "by_active_count": by_active_count(ft["log"], n_j)},
+        "split": split,
+        "all_pairs_fp64": None if ap2 is None else {
+            "interactions_per_s": ap2["fp64"]["interactions_per_s"], "gflops": ap2["fp64"]["gflops"],
+            "force_ms": ap2["fp64"]["force_ms"], "frac": ap2["fp64"]["gflops"] * 1e-3 / PEAK_FP64_TFLOPS},
         "cpu_baseline": cpu,
     }
+    if w.name == "c4":
+        line["dE_E_median"] = f.get("dE_E_median")
     print(json.dumps(line), flush=True)
 
 

@@ -118,6 +118,13 @@ def host_lib():
             getattr(L, name).restype = C.c_long
         L.hhd_get_ptcl.argtypes = [vp, _dp]
         L.hhd_profile.argtypes = [vp, _dp]
+        lp, ip = C.POINTER(C.c_long), C.POINTER(C.c_int)
+        L.hh_read_snapshot.argtypes = [C.c_char_p, C.c_int, ip, lp, _dp, _dp]
+        L.hh_write_snapshot.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_long, C.c_double, _dp, _dp]
+        L.hh_read_dump.argtypes = [C.c_char_p, C.c_int, ip, lp, _dp, _dp, _dp, lp, lp, _dp]
+        L.hh_write_dump.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_long, C.c_double, C.c_double, C.c_double,
+                                    C.c_long, C.c_long, _dp]
+        L.hh_io_error.restype = C.c_char_p
         _host = L
     return _host
 
@@ -141,6 +148,50 @@ def salpeter(n: int, seed: int, alpha: float = 2.35, mlo: float = 0.1, mhi: floa
     m = np.zeros(n)
     host_lib().hh_salpeter(n, seed, alpha, mlo, mhi, _p(m))
     return m
+
+
+# ------------------------------------------------------------------ formats
+def _io(rc):
+    if rc != 0:
+        raise HHError(host_lib().hh_io_error().decode())
+
+
+def read_snapshot(path: str, order: int):
+    """Masaki snapshot (SRC/nbodysystem.h:379-403) -> (snapid, tsys, records)."""
+    L = host_lib()
+    sid, n, t = C.c_int(), C.c_long(), C.c_double()
+    _io(L.hh_read_snapshot(path.encode(), order, C.byref(sid), C.byref(n), C.byref(t), None))
+    P = np.zeros((n.value, PSTRIDE[order]))
+    _io(L.hh_read_snapshot(path.encode(), order, C.byref(sid), C.byref(n), C.byref(t), _p(P)))
+    return sid.value, t.value, P
+
+
+def write_snapshot(path: str, order: int, snapid: int, tsys: float, P, pot):
+    """write_snapshot_masaki (SRC/nbodysystem.h:359-378)."""
+    P = np.ascontiguousarray(P, dtype=np.float64)
+    pot = np.ascontiguousarray(pot, dtype=np.float64)
+    _io(host_lib().hh_write_snapshot(path.encode(), order, snapid, len(P), tsys, _p(P), _p(pot)))
+
+
+def read_dump(path: str, order: int):
+    """restore (SRC/nbodysystem.h:428-446) -> dict(dumpid, tsys, e0, eprev,
+    nstep_tot, nbstep_tot, P)."""
+    L = host_lib()
+    did, n, t, e0, ep, a, b = C.c_int(), C.c_long(), C.c_double(), C.c_double(), C.c_double(), C.c_long(), C.c_long()
+    args = [C.byref(x) for x in (did, n, t, e0, ep, a, b)]
+    _io(L.hh_read_dump(path.encode(), order, *args, None))
+    P = np.zeros((n.value, PSTRIDE[order]))
+    _io(L.hh_read_dump(path.encode(), order, *args, _p(P)))
+    return {"dumpid": did.value, "tsys": t.value, "e0": e0.value, "eprev": ep.value, "nstep_tot": a.value,
+            "nbstep_tot": b.value, "P": P}
+
+
+def write_dump(path: str, order: int, dumpid: int, tsys: float, e0: float, eprev: float, nstep_tot: int,
+               nbstep_tot: int, P):
+    """dump (SRC/nbodysystem.h:405-427)."""
+    P = np.ascontiguousarray(P, dtype=np.float64)
+    _io(host_lib().hh_write_dump(path.encode(), order, dumpid, len(P), tsys, e0, eprev, nstep_tot, nbstep_tot,
+                                 _p(P)))
 
 
 def make_particles(order: int, mass, pos, vel, ids=None, tlast: float = 0.0) -> np.ndarray:
@@ -277,7 +328,7 @@ class System:
     SRC/nbodysystem.h) running on the HIP back end; native C++ in
     libhermite_host.so."""
 
-    PHASES = ("force", "pot", "predict", "correct", "sort", "scan", "set_jp", "total")
+    PHASES = ("force", "pot", "predict", "correct", "sort", "scan", "set_jp", "io", "total")
 
     def __init__(self, order, P, tsys, eps2, eta, eta_s, dtmax, dtmin=0.0, variant="fp64", device=0, tuning=True):
         L = host_lib()

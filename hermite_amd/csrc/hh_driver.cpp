@@ -233,7 +233,43 @@ struct Phys<8> {
     }
 };
 
-enum Phase { P_FORCE = 0, P_POT, P_PREDICT, P_CORRECT, P_SORT, P_SCAN, P_SETJP, P_TOTAL, P_NUM };
+// Profile phases in the order of the reference's Profile (SRC/nbodysystem.h:58-71)
+enum Phase { P_FORCE = 0, P_POT, P_PREDICT, P_CORRECT, P_SORT, P_SCAN, P_SETJP, P_IO, P_TOTAL, P_NUM };
+
+// Unquantised Aarseth step used when a run restarts from a dump
+// (recalc_dt: SRC/hermite4.h:95-116, SRC/hermite6.h:110-135,
+// SRC/hermite8.h:136-165): eta * (u/l)^(1/p), then dt = dt_max halved until
+// it is not above that.
+double aarseth_unquant(int order, const double *p, double eta)
+{
+    const double s0 = V3(p + ACC).norm2(), s1 = V3(p + JRK).norm2(), s2 = V3(p + SNP).norm2();
+    const double s3 = V3(p + CRK).norm2();
+    if (order == 4) {
+        const double u = std::sqrt(s0 * s2) + s1, l = std::sqrt(s1 * s3) + s2;
+        return eta * std::sqrt(u / l);
+    }
+    if (order == 6) {
+        const double s4 = V3(p + D4A).norm2(), s5 = V3(p + D5A).norm2();
+        const double u = std::sqrt(s0 * s2) + s1, l = std::sqrt(s3 * s5) + s4;
+        return eta * std::pow(u / l, 1. / 6.);
+    }
+    const double s5 = V3(p + D5A).norm2(), s6 = V3(p + D6A).norm2(), s7 = V3(p + D7A).norm2();
+    const double u = std::sqrt(s0 * s2) + s1, l = std::sqrt(s5 * s7) + s6;
+    return eta * std::pow(u / l, 1. / 10.);
+}
+
+// Derivative vectors a dump holds after id, mass, pos, vel
+// (Particle::dump, SRC/hermite4.h:45-54, SRC/hermite6.h:49-60,
+// SRC/hermite8.h:60-73): acc jrk snp crk [d4a d5a [d6a d7a]].
+constexpr int dump_vectors(int o) { return o == 4 ? 4 : o == 6 ? 6 : 8; }
+
+std::string g_io_err;
+
+int io_fail(const std::string &m)
+{
+    g_io_err = m;
+    return HH_E_ARG;
+}
 
 }  // namespace
 
@@ -242,11 +278,16 @@ struct hhd_sys {
     double eps2 = 0, eta = 0, etapow = 0, eta_s = 0, dtmax = 0, dtmin = 0, tsys = 0;
     double init_energy = 0, prev_energy = 0;
     long num_step = 0, num_bstep = 0, dtmin_hits = 0;
+    // print_stat bookkeeping (SRC/nbodysystem.h:934-969): totals are
+    // advanced only when a log line is made; marks are the step counters at
+    // the previous line
+    long num_step_tot = 0, num_bstep_tot = 0, mark_step = 0, mark_bstep = 0;
     std::vector<double> ptcl, force, pot, dtbuf, tmp;
     hh_ctx *g = nullptr;
     double prof[P_NUM] = {0};
     std::string err;
     int rc = HH_OK;
+    bool verbose = false;  // print init_step's lines as the reference does
 
     double *P(int i) { return ptcl.data() + (size_t)i * ps; }
 
@@ -364,7 +405,7 @@ struct hhd_sys {
         return k;
     }
 
-    double energy()
+    double energy(bool print = false)
     {
         const double t0 = omp_get_wtime();
         ok(hh_potential(g, eps2, pot.data()), "hh_potential");
@@ -376,6 +417,7 @@ struct hhd_sys {
         }
         ke *= 0.5;
         pe *= 0.5;
+        if (print && verbose) std::fprintf(stderr, "ke = %24.16e, pe = %24.16e, e = %24.16e\n", ke, pe, ke + pe);
         return ke + pe;
     }
 
@@ -386,8 +428,9 @@ struct hhd_sys {
             P(i)[TLAST] = tsys;
             set_jp(i);
         }
-        init_energy = prev_energy = energy();
+        init_energy = prev_energy = energy(true);
         if (order > 4) {
+            if (verbose) std::puts("INITIAL FORCE");
             ok(hh_predict_all(g, tsys), "hh_predict_all");
             ok(hh_force_first_nact(g, n, eps2, force.data(), nullptr), "hh_force_first_nact");
             for (int i = 0; i < n; i++) {
@@ -402,14 +445,17 @@ struct hhd_sys {
         double t2 = omp_get_wtime();
         prof[P_PREDICT] += t1 - t0;
         prof[P_FORCE] += t2 - t1;
+        if (verbose) std::printf("## Init N square : %f Gflops\n", ((1.0e-9 * n) * n) * flops() / (t2 - t1));
         for (int i = 0; i < n; i++) {
             assign(i);
             init_dt(i);
         }
         sort_by_level(n, dtmax);
         for (int i = 0; i < n; i++) set_jp(i);
-        init_energy = energy();
+        init_energy = energy(true);
     }
+
+    int flops() const { return order == 4 ? 60 : order == 6 ? 97 : 144; }
 
     // integrate_one_block: SRC/nbodysystem.h:692-765.  Returns nact.
     int integrate_one_block()
@@ -469,6 +515,22 @@ struct hhd_sys {
         num_bstep++;
         tsys = tnext;
         return nact;
+    }
+
+    // reinitialize_steps (SRC/nbodysystem.h:447-458): unquantised steps from
+    // the restored derivatives, dt-level partition, j-store refresh.
+    void reinitialize_steps(double eta_, double dtmax_)
+    {
+        eta = eta_;
+        dtmax = dtmax_;
+        for (int i = 0; i < n; i++) {
+            const double dta = aarseth_unquant(order, P(i), eta);
+            double dt = dtmax;
+            while (dt > dta) dt *= 0.5;
+            P(i)[DT] = dt;
+        }
+        sort_by_level(n, dtmax);
+        for (int i = 0; i < n; i++) set_jp(i);
     }
 
     void integrate_one_dtmax()
@@ -564,10 +626,243 @@ void hhd_get_ptcl(const hhd_sys *s, double *out)
     if (s) std::memcpy(out, s->ptcl.data(), sizeof(double) * s->ptcl.size());
 }
 
-// Wall seconds per phase: force, pot, predict, correct, sort, scan, set_jp, total.
+// Wall seconds per phase: force, pot, predict, correct, sort, scan, set_jp,
+// I/O, total.
 void hhd_profile(const hhd_sys *s, double *out)
 {
     if (s) std::memcpy(out, s->prof, sizeof(s->prof));
 }
+
+void hhd_add_io_time(hhd_sys *s, double seconds)
+{
+    if (s) s->prof[P_IO] += seconds;
+}
+
+// With verbose on, init_step prints the reference's lines ("INITIAL FORCE",
+// "## Init N square", the energy lines of calc_energy(true)).
+void hhd_set_verbose(hhd_sys *s, int on)
+{
+    if (s) s->verbose = on != 0;
+}
+
+const char *hh_io_error(void) { return g_io_err.c_str(); }
+
+int hhd_set_eps2(hhd_sys *s, double eps2)
+{
+    if (!s) return HH_E_ARG;
+    s->eps2 = eps2;
+    return HH_OK;
+}
+double hhd_eps2(const hhd_sys *s) { return s ? s->eps2 : NAN; }
+
+// Masaki snapshot reader (read_snapshot_masaki, SRC/nbodysystem.h:379-403):
+// "snapid nbody tsys" then per particle "id mass x y z vx vy vz pot" with
+// hexadecimal floats.  Call with recs == NULL to read the header only.
+// recs: n Particle records of the order (id, mass, pos, vel; the rest 0).
+int hh_read_snapshot(const char *path, int order, int *snapid, long *n, double *tsys, double *recs)
+{
+    if (!path || !snapid || !n || !tsys || (order != 4 && order != 6 && order != 8)) return io_fail("bad argument");
+    FILE *fp = std::fopen(path, "r");
+    if (!fp) return io_fail(std::string("cannot open ") + path);
+    int sid;
+    long nb;
+    double ts;
+    if (std::fscanf(fp, "%d %ld %lf", &sid, &nb, &ts) != 3 || nb <= 0) {
+        std::fclose(fp);
+        return io_fail(std::string("bad snapshot header in ") + path);
+    }
+    *snapid = sid;
+    *n = nb;
+    *tsys = ts;
+    if (recs) {
+        const int ps = pwords(order);
+        for (long i = 0; i < nb; i++) {
+            long id;
+            double m, x, y, z, vx, vy, vz, pot;
+            if (std::fscanf(fp, "%ld %lA %lA %lA %lA %lA %lA %lA %lA", &id, &m, &x, &y, &z, &vx, &vy, &vz, &pot) != 9) {
+                std::fclose(fp);
+                return io_fail(std::string("bad particle line in ") + path);
+            }
+            double *p = recs + (size_t)i * ps;
+            std::memset(p, 0, sizeof(double) * ps);
+            std::memcpy(p + ID, &id, 8);
+            p[MASS] = m;
+            p[POS] = x;
+            p[POS + 1] = y;
+            p[POS + 2] = z;
+            p[VEL] = vx;
+            p[VEL + 1] = vy;
+            p[VEL + 2] = vz;
+        }
+    }
+    std::fclose(fp);
+    return HH_OK;
+}
+
+// write_snapshot_masaki (SRC/nbodysystem.h:359-378) of n records with their
+// potentials.
+int hh_write_snapshot(const char *path, int order, int snapid, long n, double tsys, const double *recs,
+                      const double *pot)
+{
+    if (!path || !recs || !pot || n <= 0 || (order != 4 && order != 6 && order != 8)) return io_fail("bad argument");
+    FILE *fp = std::fopen(path, "w");
+    if (!fp) return io_fail(std::string("cannot create ") + path);
+    const int ps = pwords(order);
+    std::fprintf(fp, "%d\n%ld\n%f\n", snapid, n, tsys);
+    for (long i = 0; i < n; i++) {
+        const double *p = recs + (size_t)i * ps;
+        long id;
+        std::memcpy(&id, p + ID, 8);
+        std::fprintf(fp, "%6ld    %A    %A %A %A    %A %A %A    %A\n", id, p[MASS], p[POS], p[POS + 1], p[POS + 2],
+                     p[VEL], p[VEL + 1], p[VEL + 2], pot[i]);
+    }
+    return std::fclose(fp) == 0 ? HH_OK : io_fail(std::string("write error on ") + path);
+}
+
+// dump (SRC/nbodysystem.h:405-427 with Particle::dump): header
+// "order dumpid / nbody / tsys (%20.16f) / init_energy prev_energy (%A) /
+// num_step_tot num_bstep_tot", then id, mass, pos, vel and the derivative
+// vectors of the order in hexadecimal floats.
+int hh_write_dump(const char *path, int order, int dumpid, long n, double tsys, double e0, double eprev,
+                  long nstep_tot, long nbstep_tot, const double *recs)
+{
+    if (!path || !recs || n <= 0 || (order != 4 && order != 6 && order != 8)) return io_fail("bad argument");
+    FILE *fp = std::fopen(path, "w");
+    if (!fp) return io_fail(std::string("cannot create ") + path);
+    const int ps = pwords(order);
+    std::fprintf(fp, "%d %d\n%ld\n%20.16f\n", order, dumpid, n, tsys);
+    std::fprintf(fp, "%A %A\n", e0, eprev);
+    std::fprintf(fp, "%ld %ld\n", nstep_tot, nbstep_tot);
+    for (long i = 0; i < n; i++) {
+        const double *p = recs + (size_t)i * ps;
+        long id;
+        std::memcpy(&id, p + ID, 8);
+        std::fprintf(fp, "%8ld %A", id, p[MASS]);
+        for (int v = 0; v < 2 + dump_vectors(order); v++) {
+            const double *q = p + (v == 0 ? POS : v == 1 ? VEL : ACC + 3 * (v - 2));
+            std::fprintf(fp, "   %A %A %A", q[0], q[1], q[2]);
+        }
+        std::fprintf(fp, "\n");
+    }
+    return std::fclose(fp) == 0 ? HH_OK : io_fail(std::string("write error on ") + path);
+}
+
+// restore (SRC/nbodysystem.h:428-446): the dump's header and records, with
+// tlast = tsys and dt = 0 (reinitialize_steps sets it).  recs == NULL reads
+// the header only.
+int hh_read_dump(const char *path, int order, int *dumpid, long *n, double *tsys, double *e0, double *eprev,
+                 long *nstep_tot, long *nbstep_tot, double *recs)
+{
+    if (!path || !dumpid || !n || !tsys || !e0 || !eprev || !nstep_tot || !nbstep_tot ||
+        (order != 4 && order != 6 && order != 8))
+        return io_fail("bad argument");
+    FILE *fp = std::fopen(path, "r");
+    if (!fp) return io_fail(std::string("cannot open ") + path);
+    int ord, did;
+    long nb, a, b;
+    double ts, x0, x1;
+    if (std::fscanf(fp, "%d %d %ld %lf", &ord, &did, &nb, &ts) != 4 || std::fscanf(fp, "%lA %lA", &x0, &x1) != 2 ||
+        std::fscanf(fp, "%ld %ld", &a, &b) != 2 || nb <= 0) {
+        std::fclose(fp);
+        return io_fail(std::string("bad dump header in ") + path);
+    }
+    if (ord != order) {
+        std::fclose(fp);
+        return io_fail(std::string("dump ") + path + " is of order " + std::to_string(ord));
+    }
+    *dumpid = did;
+    *n = nb;
+    *tsys = ts;
+    *e0 = x0;
+    *eprev = x1;
+    *nstep_tot = a;
+    *nbstep_tot = b;
+    if (recs) {
+        const int ps = pwords(order);
+        for (long i = 0; i < nb; i++) {
+            double *p = recs + (size_t)i * ps;
+            std::memset(p, 0, sizeof(double) * ps);
+            long id;
+            int got = std::fscanf(fp, "%ld %lA", &id, p + MASS);
+            for (int v = 0; v < 2 + dump_vectors(order); v++) {
+                double *q = p + (v == 0 ? POS : v == 1 ? VEL : ACC + 3 * (v - 2));
+                got += std::fscanf(fp, "%lA %lA %lA", q, q + 1, q + 2);
+            }
+            if (got != 2 + 3 * (2 + dump_vectors(order))) {
+                std::fclose(fp);
+                return io_fail(std::string("bad particle record in ") + path);
+            }
+            std::memcpy(p + ID, &id, 8);
+            p[TLAST] = ts;
+        }
+    }
+    std::fclose(fp);
+    return HH_OK;
+}
+
+// The driver's current records and the potentials of its last energy.
+int hhd_write_snapshot(hhd_sys *s, const char *path, int snapid)
+{
+    if (!s) return HH_E_ARG;
+    const int r = hh_write_snapshot(path, s->order, snapid, s->n, s->tsys, s->ptcl.data(), s->pot.data());
+    if (r != HH_OK && s->rc == HH_OK) s->err = g_io_err;
+    return r;
+}
+
+int hhd_dump(hhd_sys *s, const char *path, int dumpid)
+{
+    if (!s) return HH_E_ARG;
+    const int r = hh_write_dump(path, s->order, dumpid, s->n, s->tsys, s->init_energy, s->prev_energy,
+                                s->num_step_tot, s->num_bstep_tot, s->ptcl.data());
+    if (r != HH_OK && s->rc == HH_OK) s->err = g_io_err;
+    return r;
+}
+
+// State carried by a dump besides the records: energies and step totals.
+int hhd_set_restart_state(hhd_sys *s, double e0, double eprev, long nstep_tot, long nbstep_tot)
+{
+    if (!s) return HH_E_ARG;
+    s->init_energy = e0;
+    s->prev_energy = eprev;
+    s->num_step_tot = nstep_tot;
+    s->num_bstep_tot = nbstep_tot;
+    return HH_OK;
+}
+
+int hhd_reinitialize_steps(hhd_sys *s, double eta, double dtmax)
+{
+    if (!s || !s->g) return HH_E_STATE;
+    s->reinitialize_steps(eta, dtmax);
+    return s->rc;
+}
+
+// print_stat (SRC/nbodysystem.h:934-969) without the printing: energy, the
+// interval and total step counters, then the interval counters restart.
+// out: tsys, nbstep, nstep, nbstep_tot, nstep_tot, nact_avr, de_loc, de_glo,
+// energy.
+int hhd_stat(hhd_sys *s, double *out)
+{
+    if (!s || !s->g || !out) return HH_E_ARG;
+    const double e = s->energy();
+    const long ns = s->num_step - s->mark_step, nbs = s->num_bstep - s->mark_bstep;
+    s->num_step_tot += ns;
+    s->num_bstep_tot += nbs;
+    out[0] = s->tsys;
+    out[1] = (double)nbs;
+    out[2] = (double)ns;
+    out[3] = (double)s->num_bstep_tot;
+    out[4] = (double)s->num_step_tot;
+    out[5] = double(ns) / double(nbs);
+    out[6] = (s->prev_energy - e) / s->init_energy;
+    out[7] = (s->init_energy - e) / s->init_energy;
+    out[8] = e;
+    s->prev_energy = e;
+    s->mark_step = s->num_step;
+    s->mark_bstep = s->num_bstep;
+    return s->rc;
+}
+
+long hhd_num_step_tot(const hhd_sys *s) { return s ? s->num_step_tot : -1; }
+long hhd_num_bstep_tot(const hhd_sys *s) { return s ? s->num_bstep_tot : -1; }
 
 }  // extern "C"

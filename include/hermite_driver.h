@@ -51,9 +51,57 @@ long hhd_num_bstep(const hhd_sys *s);
 long hhd_dtmin_hits(const hhd_sys *s);
 /* The n Particle records in the driver's current (sorted) order. */
 void hhd_get_ptcl(const hhd_sys *s, double *out);
-/* Wall seconds per phase: force, pot, predict, correct, sort, scan, set_jp,
- * total (no reference counterpart). */
+/* Wall seconds per phase in the order of the reference's Profile
+ * (SRC/nbodysystem.h:58-71): force, pot, predict, correct, sort, scan,
+ * set_jp, I/O, total.  hhd_add_io_time adds the caller's output time. */
 void hhd_profile(const hhd_sys *s, double *out);
+void hhd_add_io_time(hhd_sys *s, double seconds);
+/* With on != 0, init_step prints the reference's lines: "INITIAL FORCE" and
+ * "## Init N square : G Gflops" on stdout, the energy lines of
+ * calc_energy(true) on stderr (SRC/nbodysystem.h:565-650). */
+void hhd_set_verbose(hhd_sys *s, int on);
+/* Softening used by later force and energy calls (set_thermal_eps,
+ * SRC/nbodysystem.h:480-484, passes 4 / (N kT) squared). */
+int hhd_set_eps2(hhd_sys *s, double eps2);
+double hhd_eps2(const hhd_sys *s);
+
+/* ---- run control and output of the reference's main loop
+ * (SRC/compile.cpp:127-216) ---- */
+
+/* reinitialize_steps (SRC/nbodysystem.h:447-458) after a restart: dt from the
+ * unquantised Aarseth step of the restored derivatives, dt-level partition,
+ * j-store refresh. */
+int hhd_reinitialize_steps(hhd_sys *s, double eta, double dtmax);
+/* Energies and step totals a dump carries (restore, SRC/nbodysystem.h:428-446). */
+int hhd_set_restart_state(hhd_sys *s, double e0, double eprev, long nstep_tot, long nbstep_tot);
+/* print_stat (SRC/nbodysystem.h:934-969) without the printing: out[9] =
+ * tsys, block steps and particle steps since the previous call, their
+ * totals, mean nact, dE local, dE global, energy. */
+int hhd_stat(hhd_sys *s, double *out);
+long hhd_num_step_tot(const hhd_sys *s);
+long hhd_num_bstep_tot(const hhd_sys *s);
+/* write_snapshot_masaki / dump of the driver's current state. */
+int hhd_write_snapshot(hhd_sys *s, const char *path, int snapid);
+int hhd_dump(hhd_sys *s, const char *path, int dumpid);
+
+/* ---- file formats (host only, no device needed) ----
+ * Masaki snapshot (SRC/nbodysystem.h:359-403): "snapid\nnbody\ntsys\n" then
+ * "id mass x y z vx vy vz pot" per particle in %A hexadecimal floats.
+ * Dump (SRC/nbodysystem.h:405-446, Particle::dump/restore of
+ * SRC/hermite{4,6,8}.h): "order dumpid\nnbody\ntsys(%20.16f)\n",
+ * "init_energy prev_energy" (%A), "num_step_tot num_bstep_tot", then id, mass,
+ * pos, vel, acc, jrk, snp, crk [, d4a, d5a [, d6a, d7a]] per particle.
+ * recs are reference Particle records of the order; the readers take
+ * recs == NULL to read the header only.  Errors return HH_E_ARG with the
+ * text in hh_io_error(). */
+int hh_read_snapshot(const char *path, int order, int *snapid, long *n, double *tsys, double *recs);
+int hh_write_snapshot(const char *path, int order, int snapid, long n, double tsys, const double *recs,
+                      const double *pot);
+int hh_read_dump(const char *path, int order, int *dumpid, long *n, double *tsys, double *e0, double *eprev,
+                 long *nstep_tot, long *nbstep_tot, double *recs);
+int hh_write_dump(const char *path, int order, int dumpid, long n, double tsys, double e0, double eprev,
+                  long nstep_tot, long nbstep_tot, const double *recs);
+const char *hh_io_error(void);
 
 /* ---- initial conditions ---- */
 
