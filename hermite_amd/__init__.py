@@ -81,6 +81,7 @@ def hip_lib():
         L.hhr_energy.argtypes = [vp, C.c_int, _dp]
         L.hhr_set_timing.argtypes = [vp, C.c_int]
         L.hhr_set_plan.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.hhr_set_split_waves.argtypes = [vp, C.c_int, C.c_int]
         L.hhr_force_timing.argtypes = [vp, _dp, C.POINTER(C.c_longlong)]
         L.hhr_step_log.argtypes = [vp, C.c_longlong, C.POINTER(C.c_int), C.POINTER(C.c_float),
                                    C.POINTER(C.c_float)]
@@ -470,6 +471,11 @@ class Resident:
     def set_plan(self, jpar_max=64, smax=256, task_target=4096, jpar_target=256):
         """Work split of the persistent force kernel (see hhr_set_plan)."""
         self._check(hip_lib().hhr_set_plan(self.h, jpar_max, smax, task_target, jpar_target), "hhr_set_plan")
+
+    def set_split_waves(self, wj4_max=1024, wj2_max=2048):
+        """Active-count thresholds below which i-parallel tasks split each
+        j-tile over 4 (2) waves (see hhr_set_split_waves)."""
+        self._check(hip_lib().hhr_set_split_waves(self.h, wj4_max, wj2_max), "hhr_set_split_waves")
 
     def set_timing(self, on=True):
         """Bracket every force launch with HIP events (resets the totals);

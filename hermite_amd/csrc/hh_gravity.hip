@@ -1034,6 +1034,17 @@ int hhr_set_plan(hhr_ctx *c, int jpar_max, int smax, int task_target, int jpar_t
     return HH_OK;
 }
 
+int hhr_set_split_waves(hhr_ctx *c, int wj4_max, int wj2_max)
+{
+    if (!c) return HH_E_ARG;
+    RCHK(c, hipSetDevice(c->device));
+    RCHK(c, hipStreamSynchronize(c->stream));
+    const int v[2] = {wj4_max, wj2_max};
+    RCHK(c, hipMemcpy((char *)c->glob + offsetof(RGlob, plan) + offsetof(RPlan, wj4_max), v, sizeof(v),
+                      hipMemcpyHostToDevice));
+    return HH_OK;
+}
+
 int hhr_set_timing(hhr_ctx *c, int on)
 {
     if (!c) return HH_E_ARG;

@@ -48,8 +48,12 @@ constexpr int R_SLOTS = 32;      // words a block-time minimum is spread over
 // j-slices.  A block step aims at task_target i-parallel tasks
 // (jpar_target j-parallel ones) over all clusters, shared in proportion to
 // each cluster's nact * n.
+// Active counts up to wj4_max (wj2_max) split each tile's j over 4 (2) of a
+// workgroup's waves, so an i-parallel group is 64 (128) i per lane-slot
+// row instead of 256: short lists waste fewer lanes.
 struct RPlan {
     int jpar_max = 64, smax = 256, task_target = 4096, jpar_target = 256;
+    int wj4_max = 1024, wj2_max = 2048;
 };
 
 struct RClu {
@@ -63,6 +67,7 @@ struct RClu {
     int off, n, ntiles;  // particles [off, off + n), tiles from off / TJ
     int nact;
     int mode;            // 0 idle, 1 i-parallel tiles, 2 j-parallel groups
+    int wj;              // mode 1: waves sharing each tile's j (1, 2 or 4)
     int ngrp, nslice, tps, jchunk;
     int task0, ntask, act0, ipad;
     long long pofs;      // partials part[pofs + (s*NC + c)*ipad + ii]
@@ -402,7 +407,7 @@ __global__ __launch_bounds__(NT) void r_plan_kernel(RArgs a)
 #pragma unroll
     for (int q = 0; q < NW; q++) total += s_wsum[q];
 
-    const int gs = NT * r_ni(exact ? EXACT : FP64);
+    const int nil = 64 * r_ni(exact ? EXACT : FP64);  // i per wave of a mode-1 task
     int my_tasks = 0, my_act = 0, my_live = 0;
     long long my_part = 0;
     for (int k = k0; k < k1; k++) {
@@ -424,6 +429,9 @@ __global__ __launch_bounds__(NT) void r_plan_kernel(RArgs a)
         if (exact || c.nact > pl.jpar_max) {
             const int budget = max(1, (int)ceil(pl.task_target * share));
             c.mode = 1;
+            // EXACT sums each tile's j in index order on one wave
+            c.wj = exact ? 1 : c.nact <= pl.wj4_max ? 4 : c.nact <= pl.wj2_max ? 2 : 1;
+            const int gs = nil * (NT / 64 / c.wj);
             c.ngrp = (c.nact + gs - 1) / gs;
             const int ns = exact ? 1 : min(min(c.ntiles, pl.smax), max(1, (budget + c.ngrp - 1) / c.ngrp));
             c.tps = (c.ntiles + ns - 1) / ns;
@@ -544,20 +552,26 @@ __global__ __launch_bounds__(NT) void r_force_kernel(RArgs a)
         const double *src = a.jpr;
 
         if (c.mode == 1) {
-            // ---- i in lanes (NI per lane), j-tiles of the slice through LDS
-            constexpr int NI = r_ni(MODE), GS = NT * NI;
+            // ---- i in lanes (NI per lane), j-tiles of the slice through LDS.
+            // The 4 waves are wi_n rows of i by wj columns of j: wave
+            // (wi, wjx) takes i rows wi and the tile's j quarter/half wjx.
+            constexpr int NI = r_ni(MODE);
+            const int wj = c.wj, wi_n = NT / 64 / wj;
+            const int wi = wave % wi_n, wjx = wave / wi_n;
+            const int GS = 64 * wi_n * NI;
             int ii[NI];
             double pi[NI][Q];
 #pragma unroll
             for (int k = 0; k < NI; k++) {
-                ii[k] = grp * GS + k * NT + tid;
+                ii[k] = grp * GS + k * (64 * wi_n) + wi * 64 + lane;
                 const int gi = a.ilist[c.off + min(ii[k], c.nact - 1)];
 #pragma unroll
                 for (int w = 0; w < Q; w++) pi[k][w] = src[(size_t)w * npad + gi];
             }
             // a wave whose lanes are all past the active list only helps
             // stage the tiles; its SIMD slots go to the other waves
-            const bool busy = grp * GS + (tid & ~63) < c.nact;
+            const bool busy = grp * GS + wi * 64 < c.nact;
+            const int jlo = wjx * (TJ / wj), jhi = jlo + TJ / wj;
             double A[NI][NC];
 #pragma unroll
             for (int k = 0; k < NI; k++)
@@ -588,7 +602,7 @@ __global__ __launch_bounds__(NT) void r_force_kernel(RArgs a)
                     }
                 } else if constexpr (MODE == FP64) {
 #pragma unroll 2
-                    for (int jj = 0; jj < TJ; jj++) {
+                    for (int jj = jlo; jj < jhi; jj++) {
                         double q[L];
 #pragma unroll
                         for (int w = 0; w < L; w++) q[w] = sj[jj * L + w];
@@ -596,7 +610,7 @@ __global__ __launch_bounds__(NT) void r_force_kernel(RArgs a)
                         for (int k = 0; k < NI; k++) body_fp64<ORDER, false>(pi[k], q, eps2, false, 0, A[k]);
                     }
                 } else {
-                    for (int jb = 0; jb < TJ; jb += FOLD) {
+                    for (int jb = jlo; jb < jhi; jb += FOLD) {
                         float Af[NI][NC];
 #pragma unroll
                         for (int k = 0; k < NI; k++)
@@ -617,13 +631,36 @@ __global__ __launch_bounds__(NT) void r_force_kernel(RArgs a)
                     }
                 }
             }
-            double *part = a.part + c.pofs;
+            if (wj > 1) {
+                // j columns 1.. hand their sums to column 0 through the
+                // tile buffer, one i per lane at a time; column 0 adds them
+                // in column order
+                static_assert((NT / 64 - 1) * NC * 64 <= TJ * L, "tile buffer too small for the column sums");
 #pragma unroll
-            for (int k = 0; k < NI; k++)
-                if (ii[k] < c.nact) {
+                for (int k = 0; k < NI; k++) {
+                    __syncthreads();
+                    if (wjx > 0) {
 #pragma unroll
-                    for (int q = 0; q < NC; q++) part[((size_t)slice * NC + q) * c.ipad + ii[k]] = A[k][q];
+                        for (int q = 0; q < NC; q++) sj[(((wjx - 1) * wi_n + wi) * NC + q) * 64 + lane] = A[k][q];
+                    }
+                    __syncthreads();
+                    if (wjx == 0) {
+                        for (int r = 1; r < wj; r++) {
+#pragma unroll
+                            for (int q = 0; q < NC; q++) A[k][q] += sj[(((r - 1) * wi_n + wi) * NC + q) * 64 + lane];
+                        }
+                    }
                 }
+            }
+            double *part = a.part + c.pofs;
+            if (wjx == 0) {
+#pragma unroll
+                for (int k = 0; k < NI; k++)
+                    if (ii[k] < c.nact) {
+#pragma unroll
+                        for (int q = 0; q < NC; q++) part[((size_t)slice * NC + q) * c.ipad + ii[k]] = A[k][q];
+                    }
+            }
         } else if constexpr (MODE != EXACT) {  // EXACT always plans mode 1
             // ---- j in lanes, R_GI i-particles per group
             int gi[R_GI];
