@@ -18,10 +18,13 @@ import numpy as np
 
 from .build import HIP_LIB, HOST_LIB, build  # noqa: F401
 
-PSTRIDE = {4: 24, 6: 28, 8: 36}   # reference Particle, doubles
-FSTRIDE = {4: 6, 6: 9, 8: 12}     # reference Force, doubles
-JPSTRIDE = {4: 14, 6: 20, 8: 26}  # device j-particle record, doubles
-FLOPS = {4: 60, 6: 97, 8: 144}    # conventional flops per interaction (SRC/hermite{4,6,8}.h)
+# Order code 9 (ORDER_8DD) is the double-double 8th-order build
+# (SRC/hermite8dd.h): positions as posH + posL.
+ORDER_8DD = 9
+PSTRIDE = {4: 24, 6: 28, 8: 36, 9: 40}   # reference Particle, doubles
+FSTRIDE = {4: 6, 6: 9, 8: 12, 9: 12}     # reference Force, doubles
+JPSTRIDE = {4: 14, 6: 20, 8: 26, 9: 29}  # device j-particle record, doubles
+FLOPS = {4: 60, 6: 97, 8: 144, 9: 144}   # conventional flops per interaction (SRC/hermite{4,6,8}.h)
 
 VARIANTS = {"fp64": 0, "split": 1, "exact": 2}
 
@@ -204,7 +207,10 @@ def make_particles(order: int, mass, pos, vel, ids=None, tlast: float = 0.0) -> 
     P[:, 1] = mass
     P[:, 2] = tlast
     P[:, 4:7] = pos
-    P[:, 7:10] = vel
+    if order == ORDER_8DD:  # posH = pos, posL = 0 (SRC/hermite8dd.h:64-78)
+        P[:, 10:13] = vel
+    else:
+        P[:, 7:10] = vel
     return P
 
 
@@ -257,13 +263,18 @@ class Gravity:
         js = js.reshape(self.nbody, JPSTRIDE[self.order])
         js[:n, 0:3] = P[:, 4:7]
         js[:n, 3] = P[:, 1]
-        js[:n, 4:7] = P[:, 7:10]
         js[:n, 7] = P[:, 2]
-        js[:n, 8:14] = P[:, 10:16]
-        if self.order >= 6:
-            js[:n, 14:20] = P[:, 16:22]
-        if self.order == 8:
-            js[:n, 20:26] = P[:, 22:28]
+        if self.order == ORDER_8DD:  # posL after posH in the Particle, last in the j-store
+            js[:n, 4:7] = P[:, 10:13]
+            js[:n, 8:26] = P[:, 13:31]
+            js[:n, 26:29] = P[:, 7:10]
+        else:
+            js[:n, 4:7] = P[:, 7:10]
+            js[:n, 8:14] = P[:, 10:16]
+            if self.order >= 6:
+                js[:n, 14:20] = P[:, 16:22]
+            if self.order == 8:
+                js[:n, 20:26] = P[:, 22:28]
         self._check(hip_lib().hh_mark_dirty(self.h, 0, n), "hh_mark_dirty")
 
     def predict_all(self, tsys: float):

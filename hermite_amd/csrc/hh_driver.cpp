@@ -46,8 +46,14 @@ inline V3 operator*(double s, const V3 &v) { return v * s; }
 // Particle record words (the reference Particle, SRC/hermite8.h:31-45)
 enum { ID = 0, MASS = 1, TLAST = 2, DT = 3, POS = 4, VEL = 7, ACC = 10, JRK = 13, SNP = 16, CRK = 19,
        D4A = 22, D5A = 25, D6A = 28, D7A = 31 };
-constexpr int pwords(int o) { return o == 4 ? 24 : o == 6 ? 28 : 36; }
+// Order code 9: the double-double 8th-order build (SRC/hermite8dd.h), whose
+// Particle has posL after posH and is 40 words long (SRC/hermite8dd.h:44-78).
+enum { DD_POSH = 4, DD_POSL = 7, DD_VEL = 10, DD_ACC = 13, DD_JRK = 16, DD_SNP = 19, DD_CRK = 22,
+       DD_D4A = 25, DD_D5A = 28, DD_D6A = 31, DD_D7A = 34 };
+constexpr int pwords(int o) { return o == 4 ? 24 : o == 6 ? 28 : o == 8 ? 36 : 40; }
 constexpr int fwords(int o) { return o == 4 ? 6 : o == 6 ? 9 : 12; }
+constexpr int vel_word(int o) { return o == 9 ? DD_VEL : VEL; }
+constexpr bool valid_order(int o) { return o == 4 || o == 6 || o == 8 || o == 9; }
 
 // pow_one_nth_quant<N>: 2^floor(log2(x)/N) from the exponent bits
 // (SRC/pownth.h:3-19).
@@ -234,6 +240,83 @@ struct Phys<8> {
 };
 
 // Profile phases in the order of the reference's Profile (SRC/nbodysystem.h:58-71)
+// Double-double 8th order: SRC/hermite8dd.h:119-239.  The 8th-order
+// formulas on the shifted offsets; the corrector adds the position increment
+// to posH + posL with dd_accumulate (SRC/hermite8dd.h:4-24).
+template <>
+struct Phys<9> {
+    static void assign(double *p, const double *f)
+    {
+        V3(f).store(p + DD_ACC); V3(f + 3).store(p + DD_JRK); V3(f + 6).store(p + DD_SNP); V3(f + 9).store(p + DD_CRK);
+    }
+    static void init_dt(double *p, double eta_s, double dtmax)
+    {
+        const double s0 = V3(p + DD_ACC).norm2(), s1 = V3(p + DD_JRK).norm2();
+        const double s2 = V3(p + DD_SNP).norm2(), s3 = V3(p + DD_CRK).norm2();
+        const double u = std::sqrt(s0 * s2) + s1, l = std::sqrt(s1 * s3) + s2;
+        const double dtnat = eta_s * std::sqrt(u / l);
+        double dt = 0.25 * dtmax;
+        while (dt > dtnat) dt *= 0.5;
+        p[DT] = dt;
+    }
+    static double step(const double *p, double etapow)
+    {
+        const double s0 = V3(p + DD_ACC).norm2(), s1 = V3(p + DD_JRK).norm2(), s2 = V3(p + DD_SNP).norm2();
+        const double s5 = V3(p + DD_D5A).norm2(), s6 = V3(p + DD_D6A).norm2(), s7 = V3(p + DD_D7A).norm2();
+        const double u = std::sqrt(s0 * s2) + s1, l = std::sqrt(s5 * s7) + s6;
+        return pow_nth_quant<10>(etapow * (u / l));
+    }
+    static void dd_accumulate(double &xh, double &xl, double dx)
+    {
+        const double yh = xh + dx;
+        const double res = dx - (yh - xh);
+        const double yl = xl + res;
+        const double zh = yh + yl;
+        const double zl = yl - (zh - yh);
+        xh = zh;
+        xl = zl;
+    }
+    static void correct(double *p, const double *f, double etapow, double dtlim)
+    {
+        const double dt = p[DT];
+        const double h = 0.5 * dt;
+        const double hinv = 2.0 / dt;
+        const V3 fa(f), fj(f + 3), fs(f + 6), fc(f + 9);
+        const V3 acc(p + DD_ACC), jrk(p + DD_JRK), snp(p + DD_SNP), crk(p + DD_CRK), vel(p + DD_VEL);
+        const V3 Ap = fa + acc, Am = fa - acc;
+        const V3 Jp = (fj + jrk) * h, Jm = (fj - jrk) * h;
+        const V3 Sp = (fs + snp) * (h * h), Sm = (fs - snp) * (h * h);
+        const V3 Cp = (fc + crk) * (h * h * h), Cm = (fc - crk) * (h * h * h);
+        const V3 vel1 = vel + h * (Ap + (-3. / 7.) * Jm + (2. / 21) * Sp - (1. / 105.) * Cm);
+        const V3 dpos = h * ((vel + vel1) + h * ((-3. / 7.) * Am + (2. / 21) * Jp - (1. / 105.) * Sm));
+        dd_accumulate(p[DD_POSH], p[DD_POSL], dpos.x);
+        dd_accumulate(p[DD_POSH + 1], p[DD_POSL + 1], dpos.y);
+        dd_accumulate(p[DD_POSH + 2], p[DD_POSL + 2], dpos.z);
+        vel1.store(p + DD_VEL);
+        fa.store(p + DD_ACC);
+        fj.store(p + DD_JRK);
+        fs.store(p + DD_SNP);
+        fc.store(p + DD_CRK);
+        p[TLAST] += dt;
+        const double hinv2 = hinv * hinv, hinv3 = hinv2 * hinv, hinv4 = hinv2 * hinv2;
+        const double hinv5 = hinv2 * hinv3, hinv6 = hinv3 * hinv3, hinv7 = hinv3 * hinv4;
+        V3 d4a = (hinv4 * 24. / 32.) * (-5. * Jm + 5. * Sp - Cm);
+        V3 d5a = (hinv5 * 120. / 32.) * (21. * Am - 21. * Jp + 8. * Sm - Cp);
+        V3 d6a = (hinv6 * 720. / 32.) * (Jm - Sp + Cm / 3.);
+        const V3 d7a = (hinv7 * 5040. / 32.) * (-5. * Am + 5. * Jp - 2. * Sm + Cp / 3.);
+        const double h2 = 0.5 * h, h3 = (1. / 3.) * h;
+        d4a = d4a + h * (d5a + h2 * (d6a + h3 * d7a));
+        d5a = d5a + h * (d6a + h2 * d7a);
+        d6a = d6a + h * d7a;
+        d4a.store(p + DD_D4A);
+        d5a.store(p + DD_D5A);
+        d6a.store(p + DD_D6A);
+        d7a.store(p + DD_D7A);
+        const double dtq = step(p, etapow);
+        p[DT] = dtq < dtlim ? dtq : dtlim;
+    }
+};
+
 enum Phase { P_FORCE = 0, P_POT, P_PREDICT, P_CORRECT, P_SORT, P_SCAN, P_SETJP, P_IO, P_TOTAL, P_NUM };
 
 // Unquantised Aarseth step used when a run restarts from a dump
@@ -242,6 +325,12 @@ enum Phase { P_FORCE = 0, P_POT, P_PREDICT, P_CORRECT, P_SORT, P_SCAN, P_SETJP, 
 // it is not above that.
 double aarseth_unquant(int order, const double *p, double eta)
 {
+    if (order == 9) {  // SRC/hermite8dd.h:164-186, the 8th-order step on shifted offsets
+        const double s0 = V3(p + DD_ACC).norm2(), s1 = V3(p + DD_JRK).norm2(), s2 = V3(p + DD_SNP).norm2();
+        const double s5 = V3(p + DD_D5A).norm2(), s6 = V3(p + DD_D6A).norm2(), s7 = V3(p + DD_D7A).norm2();
+        const double u = std::sqrt(s0 * s2) + s1, l = std::sqrt(s5 * s7) + s6;
+        return eta * std::pow(u / l, 1. / 10.);
+    }
     const double s0 = V3(p + ACC).norm2(), s1 = V3(p + JRK).norm2(), s2 = V3(p + SNP).norm2();
     const double s3 = V3(p + CRK).norm2();
     if (order == 4) {
@@ -258,10 +347,14 @@ double aarseth_unquant(int order, const double *p, double eta)
     return eta * std::pow(u / l, 1. / 10.);
 }
 
-// Derivative vectors a dump holds after id, mass, pos, vel
-// (Particle::dump, SRC/hermite4.h:45-54, SRC/hermite6.h:49-60,
-// SRC/hermite8.h:60-73): acc jrk snp crk [d4a d5a [d6a d7a]].
-constexpr int dump_vectors(int o) { return o == 4 ? 4 : o == 6 ? 6 : 8; }
+// Vectors a dump holds after id and mass (Particle::dump,
+// SRC/hermite4.h:45-54, SRC/hermite6.h:49-60, SRC/hermite8.h:60-73,
+// SRC/hermite8dd.h:80-94): pos vel acc jrk snp crk [d4a d5a [d6a d7a]], and
+// posH posL vel ... d7a for the double-double build, i.e. the record's
+// 3-vectors from word 4 in order.
+constexpr int dump_vectors(int o) { return o == 4 ? 6 : o == 6 ? 8 : o == 8 ? 10 : 11; }
+inline double *dump_vec(double *p, int v) { return p + POS + 3 * v; }
+inline const double *dump_vec(const double *p, int v) { return p + POS + 3 * v; }
 
 std::string g_io_err;
 
@@ -308,7 +401,8 @@ struct hhd_sys {
         switch (order) {
         case 4: Phys<4>::assign(P(i), f); break;
         case 6: Phys<6>::assign(P(i), f); break;
-        default: Phys<8>::assign(P(i), f); break;
+        case 8: Phys<8>::assign(P(i), f); break;
+        default: Phys<9>::assign(P(i), f); break;
         }
     }
     void init_dt(int i)
@@ -316,7 +410,8 @@ struct hhd_sys {
         switch (order) {
         case 4: Phys<4>::init_dt(P(i), eta_s, dtmax); break;
         case 6: Phys<6>::init_dt(P(i), eta_s, dtmax); break;
-        default: Phys<8>::init_dt(P(i), eta_s, dtmax); break;
+        case 8: Phys<8>::init_dt(P(i), eta_s, dtmax); break;
+        default: Phys<9>::init_dt(P(i), eta_s, dtmax); break;
         }
     }
     void correct(int i, double dtlim)
@@ -325,7 +420,8 @@ struct hhd_sys {
         switch (order) {
         case 4: Phys<4>::correct(P(i), f, etapow, dtlim); break;
         case 6: Phys<6>::correct(P(i), f, etapow, dtlim); break;
-        default: Phys<8>::correct(P(i), f, etapow, dtlim); break;
+        case 8: Phys<8>::correct(P(i), f, etapow, dtlim); break;
+        default: Phys<9>::correct(P(i), f, etapow, dtlim); break;
         }
     }
 
@@ -412,7 +508,7 @@ struct hhd_sys {
         prof[P_POT] += omp_get_wtime() - t0;
         double ke = 0.0, pe = 0.0;
         for (int i = 0; i < n; i++) {
-            ke += P(i)[MASS] * V3(P(i) + VEL).norm2();
+            ke += P(i)[MASS] * V3(P(i) + vel_word(order)).norm2();
             pe += P(i)[MASS] * pot[i];
         }
         ke *= 0.5;
@@ -490,7 +586,8 @@ struct hhd_sys {
                 P(i)[DT] = dtmin;
                 hits++;
             }
-            if (!std::isfinite(P(i)[POS] + P(i)[POS + 1] + P(i)[POS + 2] + P(i)[VEL] + P(i)[VEL + 1] + P(i)[VEL + 2]))
+            const int vw = vel_word(order);
+            if (!std::isfinite(P(i)[POS] + P(i)[POS + 1] + P(i)[POS + 2] + P(i)[vw] + P(i)[vw + 1] + P(i)[vw + 2]))
                 bad++;
             dtbuf[i] = P(i)[DT];
         }
@@ -549,7 +646,7 @@ extern "C" {
 hhd_sys *hhd_create(int order, int n, const double *recs, double tsys, double eps2, double eta, double eta_s,
                     double dtmax, double dtmin, int variant, int device)
 {
-    if ((order != 4 && order != 6 && order != 8) || n <= 0 || !recs) return nullptr;
+    if (!valid_order(order) || n <= 0 || !recs) return nullptr;
     hhd_sys *s = new hhd_sys();
     s->order = order;
     s->n = n;
@@ -561,7 +658,7 @@ hhd_sys *hhd_create(int order, int n, const double *recs, double tsys, double ep
     s->dtmax = dtmax;
     s->dtmin = dtmin;
     s->tsys = tsys;
-    s->etapow = std::pow(eta, double(2 * (order - 3)));
+    s->etapow = std::pow(eta, double(2 * ((order == 9 ? 8 : order) - 3)));
     s->ptcl.assign(recs, recs + (size_t)n * s->ps);
     s->force.assign((size_t)n * s->fs, 0.0);
     s->pot.assign(n, 0.0);
@@ -661,7 +758,7 @@ double hhd_eps2(const hhd_sys *s) { return s ? s->eps2 : NAN; }
 // recs: n Particle records of the order (id, mass, pos, vel; the rest 0).
 int hh_read_snapshot(const char *path, int order, int *snapid, long *n, double *tsys, double *recs)
 {
-    if (!path || !snapid || !n || !tsys || (order != 4 && order != 6 && order != 8)) return io_fail("bad argument");
+    if (!path || !snapid || !n || !tsys || !valid_order(order)) return io_fail("bad argument");
     FILE *fp = std::fopen(path, "r");
     if (!fp) return io_fail(std::string("cannot open ") + path);
     int sid;
@@ -690,9 +787,10 @@ int hh_read_snapshot(const char *path, int order, int *snapid, long *n, double *
             p[POS] = x;
             p[POS + 1] = y;
             p[POS + 2] = z;
-            p[VEL] = vx;
-            p[VEL + 1] = vy;
-            p[VEL + 2] = vz;
+            const int vw = vel_word(order);  // posH = pos, posL = 0 for the double-double build
+            p[vw] = vx;
+            p[vw + 1] = vy;
+            p[vw + 2] = vz;
         }
     }
     std::fclose(fp);
@@ -704,7 +802,7 @@ int hh_read_snapshot(const char *path, int order, int *snapid, long *n, double *
 int hh_write_snapshot(const char *path, int order, int snapid, long n, double tsys, const double *recs,
                       const double *pot)
 {
-    if (!path || !recs || !pot || n <= 0 || (order != 4 && order != 6 && order != 8)) return io_fail("bad argument");
+    if (!path || !recs || !pot || n <= 0 || !valid_order(order)) return io_fail("bad argument");
     FILE *fp = std::fopen(path, "w");
     if (!fp) return io_fail(std::string("cannot create ") + path);
     const int ps = pwords(order);
@@ -713,8 +811,14 @@ int hh_write_snapshot(const char *path, int order, int snapid, long n, double ts
         const double *p = recs + (size_t)i * ps;
         long id;
         std::memcpy(&id, p + ID, 8);
-        std::fprintf(fp, "%6ld    %A    %A %A %A    %A %A %A    %A\n", id, p[MASS], p[POS], p[POS + 1], p[POS + 2],
-                     p[VEL], p[VEL + 1], p[VEL + 2], pot[i]);
+        // the double-double build writes posH + posL (the intent of
+        // SRC/nbodysystem.h:366-370, whose misspelt EIGTHDD test makes the
+        // reference build read a pos member it does not have)
+        const int vw = vel_word(order);
+        double x[3];
+        for (int k = 0; k < 3; k++) x[k] = order == 9 ? p[DD_POSH + k] + p[DD_POSL + k] : p[POS + k];
+        std::fprintf(fp, "%6ld    %A    %A %A %A    %A %A %A    %A\n", id, p[MASS], x[0], x[1], x[2], p[vw],
+                     p[vw + 1], p[vw + 2], pot[i]);
     }
     return std::fclose(fp) == 0 ? HH_OK : io_fail(std::string("write error on ") + path);
 }
@@ -726,11 +830,11 @@ int hh_write_snapshot(const char *path, int order, int snapid, long n, double ts
 int hh_write_dump(const char *path, int order, int dumpid, long n, double tsys, double e0, double eprev,
                   long nstep_tot, long nbstep_tot, const double *recs)
 {
-    if (!path || !recs || n <= 0 || (order != 4 && order != 6 && order != 8)) return io_fail("bad argument");
+    if (!path || !recs || n <= 0 || !valid_order(order)) return io_fail("bad argument");
     FILE *fp = std::fopen(path, "w");
     if (!fp) return io_fail(std::string("cannot create ") + path);
     const int ps = pwords(order);
-    std::fprintf(fp, "%d %d\n%ld\n%20.16f\n", order, dumpid, n, tsys);
+    std::fprintf(fp, "%d %d\n%ld\n%20.16f\n", order == 9 ? 8 : order, dumpid, n, tsys);
     std::fprintf(fp, "%A %A\n", e0, eprev);
     std::fprintf(fp, "%ld %ld\n", nstep_tot, nbstep_tot);
     for (long i = 0; i < n; i++) {
@@ -738,8 +842,8 @@ int hh_write_dump(const char *path, int order, int dumpid, long n, double tsys, 
         long id;
         std::memcpy(&id, p + ID, 8);
         std::fprintf(fp, "%8ld %A", id, p[MASS]);
-        for (int v = 0; v < 2 + dump_vectors(order); v++) {
-            const double *q = p + (v == 0 ? POS : v == 1 ? VEL : ACC + 3 * (v - 2));
+        for (int v = 0; v < dump_vectors(order); v++) {
+            const double *q = dump_vec(p, v);
             std::fprintf(fp, "   %A %A %A", q[0], q[1], q[2]);
         }
         std::fprintf(fp, "\n");
@@ -754,7 +858,7 @@ int hh_read_dump(const char *path, int order, int *dumpid, long *n, double *tsys
                  long *nstep_tot, long *nbstep_tot, double *recs)
 {
     if (!path || !dumpid || !n || !tsys || !e0 || !eprev || !nstep_tot || !nbstep_tot ||
-        (order != 4 && order != 6 && order != 8))
+        !valid_order(order))
         return io_fail("bad argument");
     FILE *fp = std::fopen(path, "r");
     if (!fp) return io_fail(std::string("cannot open ") + path);
@@ -766,7 +870,7 @@ int hh_read_dump(const char *path, int order, int *dumpid, long *n, double *tsys
         std::fclose(fp);
         return io_fail(std::string("bad dump header in ") + path);
     }
-    if (ord != order) {
+    if (ord != (order == 9 ? 8 : order)) {
         std::fclose(fp);
         return io_fail(std::string("dump ") + path + " is of order " + std::to_string(ord));
     }
@@ -784,11 +888,11 @@ int hh_read_dump(const char *path, int order, int *dumpid, long *n, double *tsys
             std::memset(p, 0, sizeof(double) * ps);
             long id;
             int got = std::fscanf(fp, "%ld %lA", &id, p + MASS);
-            for (int v = 0; v < 2 + dump_vectors(order); v++) {
-                double *q = p + (v == 0 ? POS : v == 1 ? VEL : ACC + 3 * (v - 2));
+            for (int v = 0; v < dump_vectors(order); v++) {
+                double *q = dump_vec(p, v);
                 got += std::fscanf(fp, "%lA %lA %lA", q, q + 1, q + 2);
             }
-            if (got != 2 + 3 * (2 + dump_vectors(order))) {
+            if (got != 2 + 3 * dump_vectors(order)) {
                 std::fclose(fp);
                 return io_fail(std::string("bad particle record in ") + path);
             }

@@ -69,10 +69,11 @@ static int fail(hh_ctx *c, int code, const std::string &msg)
 
 extern "C" {
 
-int hh_particle_words(int order) { return (order == 4 || order == 6 || order == 8) ? p_words(order) : HH_E_ARG; }
-int hh_force_words(int order) { return (order == 4 || order == 6 || order == 8) ? f_words(order) : HH_E_ARG; }
-int hh_jp_words(int order) { return (order == 4 || order == 6 || order == 8) ? jp_words(order) : HH_E_ARG; }
-int hh_pred_words(int order) { return (order == 4 || order == 6 || order == 8) ? pr_words(order) : HH_E_ARG; }
+static bool valid_order(int order) { return order == 4 || order == 6 || order == 8 || order == HH_ORDER_8DD; }
+int hh_particle_words(int order) { return valid_order(order) ? p_words(order) : HH_E_ARG; }
+int hh_force_words(int order) { return valid_order(order) ? f_words(order) : HH_E_ARG; }
+int hh_jp_words(int order) { return valid_order(order) ? jp_words(order) : HH_E_ARG; }
+int hh_pred_words(int order) { return valid_order(order) ? pr_words(order) : HH_E_ARG; }
 
 const char *hh_last_error(const hh_ctx *c) { return c ? c->err.c_str() : g_create_err.c_str(); }
 
@@ -140,6 +141,7 @@ static int sync_and_predict(hh_ctx *c, double tsys)
     case 4: e = ex ? launch_predict<4, true, PREDICT>(c, nd, tsys) : launch_predict<4, false, PREDICT>(c, nd, tsys); break;
     case 6: e = ex ? launch_predict<6, true, PREDICT>(c, nd, tsys) : launch_predict<6, false, PREDICT>(c, nd, tsys); break;
     case 8: e = ex ? launch_predict<8, true, PREDICT>(c, nd, tsys) : launch_predict<8, false, PREDICT>(c, nd, tsys); break;
+    case 9: e = ex ? launch_predict<9, true, PREDICT>(c, nd, tsys) : launch_predict<9, false, PREDICT>(c, nd, tsys); break;
     }
     if (e != hipSuccess) return fail(c, HH_E_HIP, std::string("predict launch: ") + hipGetErrorString(e));
     return HH_OK;
@@ -328,6 +330,8 @@ static int launch_force_order(hh_ctx *c, int nact, double eps2, bool pot)
     case HH_FP64:
         return pot ? launch_force_mode<ORDER, FP64, true>(c, nact, eps2) : launch_force_mode<ORDER, FP64, false>(c, nact, eps2);
     case HH_SPLIT:
+        if constexpr (ORDER == 9) return fail(c, HH_E_ARG, "no split-precision kernel for double-double positions");
+        else
         return pot ? launch_force_mode<ORDER, SPLIT, true>(c, nact, eps2) : launch_force_mode<ORDER, SPLIT, false>(c, nact, eps2);
     default:
         return pot ? launch_force_mode<ORDER, EXACT, true>(c, nact, eps2) : launch_force_mode<ORDER, EXACT, false>(c, nact, eps2);
@@ -339,13 +343,17 @@ static int launch_force(hh_ctx *c, int nact, double eps2, bool pot)
     switch (c->order) {
     case 4: return launch_force_order<4>(c, nact, eps2, pot);
     case 6: return launch_force_order<6>(c, nact, eps2, pot);
-    default: return launch_force_order<8>(c, nact, eps2, pot);
+    case 8: return launch_force_order<8>(c, nact, eps2, pot);
+    default: return launch_force_order<9>(c, nact, eps2, pot);
     }
 }
 
 static int launch_potential(hh_ctx *c, double eps2)
 {
     const Geo g{1, 1, 0};
+    if (c->order == HH_ORDER_8DD)  // hi + lo positions, self excluded by index
+        return c->variant == HH_EXACT ? launch_tile<1, EXACT, false>(c, g, c->d_jst, c->nbody, eps2, c->d_pot, nullptr)
+                                      : launch_tile<1, FP64, false>(c, g, c->d_jst, c->nbody, eps2, c->d_pot, nullptr);
     switch (c->variant) {
     case HH_FP64: return launch_tile<0, FP64, false>(c, g, c->d_jst, c->nbody, eps2, c->d_pot, nullptr);
     case HH_SPLIT: return launch_tile<0, SPLIT, false>(c, g, c->d_jst, c->nbody, eps2, c->d_pot, nullptr);
@@ -368,9 +376,11 @@ int hh_create(int order, int nbody, int variant, int device, hh_ctx **out)
 {
     if (!out) return fail(nullptr, HH_E_ARG, "out is null");
     *out = nullptr;
-    if (order != 4 && order != 6 && order != 8) return fail(nullptr, HH_E_ARG, "order must be 4, 6 or 8");
+    if (!valid_order(order)) return fail(nullptr, HH_E_ARG, "order must be 4, 6, 8 or HH_ORDER_8DD");
     if (nbody <= 0) return fail(nullptr, HH_E_ARG, "nbody must be positive");
     if (variant < HH_FP64 || variant > HH_EXACT) return fail(nullptr, HH_E_ARG, "unknown variant");
+    if (order == HH_ORDER_8DD && variant == HH_SPLIT)
+        return fail(nullptr, HH_E_ARG, "no split-precision kernel for double-double positions");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(nullptr, HH_E_NODEV, "no HIP device");
     if (device < 0 || device >= ndev) return fail(nullptr, HH_E_ARG, "device ordinal out of range");
@@ -411,7 +421,8 @@ int hh_create(int order, int nbody, int variant, int device, hh_ctx **out)
     switch (order) {
     case 4: e = launch_pad<4>(c); break;
     case 6: e = launch_pad<6>(c); break;
-    default: e = launch_pad<8>(c); break;
+    case 8: e = launch_pad<8>(c); break;
+    default: e = launch_pad<9>(c); break;
     }
     if (e != hipSuccess) return bail("pad", e);
     if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return bail("sync", e);
@@ -441,11 +452,22 @@ int hh_mark_dirty(hh_ctx *c, int lo, int hi)
 }
 
 // set_jp: SRC/hermite6-titan.h:70-97 (field copy into the pinned j-store).
+// The double-double Particle (SRC/hermite8dd.h:44-78) has posL after posH;
+// its j-store record is the 8th-order one followed by posL.
 int hh_set_jp(hh_ctx *c, int addr, const double *p)
 {
     if (!c || !p || addr < 0 || addr >= c->nbody) return HH_E_ARG;
     const int W = jp_words(c->order);
     double *d = c->h_jp + (size_t)addr * W;
+    if (c->order == HH_ORDER_8DD) {
+        d[0] = p[4];  d[1] = p[5];  d[2] = p[6];     // posH
+        d[3] = p[1];                                 // mass
+        d[4] = p[10]; d[5] = p[11]; d[6] = p[12];    // vel
+        d[7] = p[2];                                 // tlast
+        for (int w = 0; w < 18; w++) d[8 + w] = p[13 + w];  // acc jrk snp crk d4a d5a
+        d[26] = p[7]; d[27] = p[8]; d[28] = p[9];    // posL
+        return hh_mark_dirty(c, 0, addr + 1);
+    }
     d[0] = p[4];  d[1] = p[5];  d[2] = p[6];   // pos
     d[3] = p[1];                               // mass
     d[4] = p[7];  d[5] = p[8];  d[6] = p[9];   // vel

@@ -23,12 +23,26 @@ constexpr int TJ = 256;     // j per LDS tile = threads per workgroup
 constexpr int NT = 256;     // threads per workgroup
 constexpr double FAR = 1.0e10;  // position of padding particles
 
-__host__ __device__ constexpr int jp_words(int o) { return o == 4 ? 14 : o == 6 ? 20 : 26; }
-__host__ __device__ constexpr int pr_words(int o) { return o == 4 ? 7 : o == 6 ? 10 : o == 8 ? 13 : 4; }
-__host__ __device__ constexpr int f_words(int o) { return o == 4 ? 6 : o == 6 ? 9 : o == 8 ? 12 : 0; }
-__host__ __device__ constexpr int p_words(int o) { return o == 4 ? 24 : o == 6 ? 28 : 36; }
+// Order codes: 4, 6, 8; 9 is the double-double 8th-order build (positions
+// as hi + lo, SRC/hermite8dd.h); 0 and 1 are the stand-alone potential over
+// corrected positions (1 with hi + lo positions).
+//   j-store (jst) words: order 9 appends posL (26-28) to the order-8 words.
+//   predictions (jpr): order 9 is posH m vel acc jrk posL (16 words).
+__host__ __device__ constexpr int jp_words(int o) { return o == 4 ? 14 : o == 6 ? 20 : o == 8 ? 26 : 29; }
+__host__ __device__ constexpr int pr_words(int o)
+{
+    return o == 4 ? 7 : o == 6 ? 10 : o == 8 ? 13 : o == 9 ? 16 : o == 1 ? 7 : 4;
+}
+__host__ __device__ constexpr int f_words(int o) { return o == 4 ? 6 : o == 6 ? 9 : (o == 8 || o == 9) ? 12 : 0; }
+__host__ __device__ constexpr int p_words(int o) { return o == 4 ? 24 : o == 6 ? 28 : o == 8 ? 36 : 40; }
 // words per j record in LDS, even so records can be read with 16-byte loads
-__host__ __device__ constexpr int lds_words(int o) { return o == 4 ? 8 : o == 6 ? 10 : o == 8 ? 14 : 4; }
+__host__ __device__ constexpr int lds_words(int o)
+{
+    return o == 4 ? 8 : o == 6 ? 10 : o == 8 ? 14 : o == 9 ? 16 : o == 1 ? 8 : 4;
+}
+// Source word of record word w for the tile kernels: the double-double
+// potential (order 1) reads posH, mass and posL (j-store words 26-28).
+__host__ __device__ constexpr int src_word(int o, int w) { return (o == 1 && w >= 4) ? 22 + w : w; }
 
 // ------------------------------------------------------------------ rsqrt
 // 1/sqrt(x) to FP64 accuracy: an FP32 v_rsq_f32 seed y0 refined with the
@@ -81,7 +95,16 @@ __device__ __forceinline__ void predict_words(const double (&r)[jp_words(ORDER)]
             q[7 + k] = madd<EX>(dt, madd<EX>(dt2, madd<EX>(dt3, c, s), j), a);
         } else {
             const double s = r[14 + k], c = r[17 + k], d4 = r[20 + k], d5 = r[23 + k];
-            q[0 + k] = madd<EX>(dt, madd<EX>(dt2, madd<EX>(dt3, madd<EX>(dt4, madd<EX>(dt5, madd<EX>(dt6, madd<EX>(dt7, d5, d4), c), s), j), a), v), x);
+            // double-double: posH is kept and the displacement goes to posL
+            // (SRC/hermite8dd.h:291-311)
+            const double xs = ORDER == 9 ? r[26 + k] : x;
+            const double xp = madd<EX>(dt, madd<EX>(dt2, madd<EX>(dt3, madd<EX>(dt4, madd<EX>(dt5, madd<EX>(dt6, madd<EX>(dt7, d5, d4), c), s), j), a), v), xs);
+            if constexpr (ORDER == 9) {
+                q[0 + k] = x;
+                q[13 + k] = xp;
+            } else {
+                q[0 + k] = xp;
+            }
             q[4 + k] = madd<EX>(dt, madd<EX>(dt2, madd<EX>(dt3, madd<EX>(dt4, madd<EX>(dt5, madd<EX>(dt6, d5, d4), c), s), j), a), v);
             q[7 + k] = madd<EX>(dt, madd<EX>(dt2, madd<EX>(dt3, madd<EX>(dt4, madd<EX>(dt5, d5, d4), c), s), j), a);
             q[10 + k] = madd<EX>(dt, madd<EX>(dt2, madd<EX>(dt3, madd<EX>(dt4, d5, d4), c), s), j);
@@ -154,7 +177,7 @@ struct TileArgs {
 
 // Accumulator count: force words (+1 with the fused potential); ORDER 0 is
 // the stand-alone potential.
-__host__ __device__ constexpr int n_acc(int order, bool pot) { return order == 0 ? 1 : f_words(order) + (pot ? 1 : 0); }
+__host__ __device__ constexpr int n_acc(int order, bool pot) { return order <= 1 ? 1 : f_words(order) + (pot ? 1 : 0); }
 
 // ---- interaction bodies -------------------------------------------------
 // q: j record (pos[3] m vel[3] acc[3] jrk[3]); p: i record; A: accumulators
@@ -167,8 +190,11 @@ __host__ __device__ constexpr int n_acc(int order, bool pot) { return order == 0
 template <int ORDER, bool POT>
 __device__ __forceinline__ void body_exact(const double *p, const double *q, double eps2, bool self, int mask_r2, double *A)
 {
-    if constexpr (ORDER == 0) {
-        const double dx = q[0] - p[0], dy = q[1] - p[1], dz = q[2] - p[2];
+    if constexpr (ORDER <= 1) {
+        // order 1: (posHj - posHi) + (posLj - posLi), SRC/hermite8dd.h:391-413
+        const double dx = ORDER == 1 ? (q[0] - p[0]) + (q[4] - p[4]) : q[0] - p[0];
+        const double dy = ORDER == 1 ? (q[1] - p[1]) + (q[5] - p[5]) : q[1] - p[1];
+        const double dz = ORDER == 1 ? (q[2] - p[2]) + (q[6] - p[6]) : q[2] - p[2];
         const double r2 = eps2 + (dx * dx + dy * dy + dz * dz);
         const double mj = (mask_r2 ? (r2 > eps2) : !self) ? q[3] : 0.0;
         A[0] -= mj / __builtin_sqrt(r2);
@@ -177,18 +203,21 @@ __device__ __forceinline__ void body_exact(const double *p, const double *q, dou
         double dr[3], dv[3], da[3], dj[3];
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            dr[k] = q[0 + k] - p[0 + k];
+            // double-double: (posHj - posHi) + (posLj - posLi), SRC/hermite8dd.h:358
+            dr[k] = ORDER == 9 ? (q[0 + k] - p[0 + k]) + (q[13 + k] - p[13 + k]) : q[0 + k] - p[0 + k];
             dv[k] = q[4 + k] - p[4 + k];
         }
         const double r2 = eps2 + (dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
-        const double drdv = dr[0] * dv[0] + dr[1] * dv[1] + dr[2] * dv[2];
-        const double ri2 = 1.0 / r2;
-        const double sq = __builtin_sqrt(ri2);
-        const double mri3 = q[3] * ri2 * sq;
         if constexpr (POT) {
             const double mj = self ? 0.0 : q[3];
             A[f_words(ORDER)] -= mj / __builtin_sqrt(r2);
         }
+        // the double-double build skips r2 == eps2 (SRC/hermite8dd.h:364)
+        if (ORDER == 9 && r2 == eps2) return;
+        const double drdv = dr[0] * dv[0] + dr[1] * dv[1] + dr[2] * dv[2];
+        const double ri2 = 1.0 / r2;
+        const double sq = __builtin_sqrt(ri2);
+        const double mri3 = q[3] * ri2 * sq;
         if constexpr (ORDER == 4) {
             const double alpha = -3.0 * drdv * ri2;
 #pragma unroll
@@ -238,10 +267,15 @@ __device__ __forceinline__ void body_exact(const double *p, const double *q, dou
 template <int ORDER, bool POT>
 __device__ __forceinline__ void body_fp64(const double *p, const double *q, double eps2, bool self, int mask_r2, double *A)
 {
-    const double dx = q[0] - p[0], dy = q[1] - p[1], dz = q[2] - p[2];
+    // relative position; hi + lo positions for the double-double orders
+    constexpr int LO = ORDER == 9 ? 13 : 4;
+    constexpr bool DD = ORDER == 9 || ORDER == 1;
+    const double dx = DD ? (q[0] - p[0]) + (q[LO] - p[LO]) : q[0] - p[0];
+    const double dy = DD ? (q[1] - p[1]) + (q[LO + 1] - p[LO + 1]) : q[1] - p[1];
+    const double dz = DD ? (q[2] - p[2]) + (q[LO + 2] - p[LO + 2]) : q[2] - p[2];
     const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, __builtin_fma(dx, dx, eps2)));
     const double rinv = rsqrt64(r2);
-    if constexpr (ORDER == 0) {
+    if constexpr (ORDER <= 1) {
         const bool keep = mask_r2 ? (r2 > eps2) : !self;
         A[0] = __builtin_fma(keep ? -q[3] : 0.0, rinv, A[0]);
         return;
@@ -249,7 +283,8 @@ __device__ __forceinline__ void body_fp64(const double *p, const double *q, doub
         const double dvx = q[4] - p[4], dvy = q[5] - p[5], dvz = q[6] - p[6];
         const double rinv2 = rinv * rinv;
         const double mr = q[3] * rinv;
-        const double mr3 = mr * rinv2;
+        // the double-double build skips r2 == eps2 (SRC/hermite8dd.h:364)
+        const double mr3 = (ORDER == 9 && r2 == eps2) ? 0.0 : mr * rinv2;
         if constexpr (POT) A[f_words(ORDER)] = __builtin_fma(self ? 0.0 : -1.0, mr, A[f_words(ORDER)]);
         const double drdv = __builtin_fma(dz, dvz, __builtin_fma(dy, dvy, dx * dvx));
         A[0] = __builtin_fma(mr3, dx, A[0]);
@@ -278,7 +313,7 @@ __device__ __forceinline__ void body_fp64(const double *p, const double *q, doub
             A[6] = __builtin_fma(mr3, ux, A[6]);
             A[7] = __builtin_fma(mr3, uy, A[7]);
             A[8] = __builtin_fma(mr3, uz, A[8]);
-            if constexpr (ORDER == 8) {
+            if constexpr (ORDER >= 8) {
                 const double djx = q[10] - p[10], djy = q[11] - p[11], djz = q[12] - p[12];
                 const double dvda = __builtin_fma(dvz, daz, __builtin_fma(dvy, day, dvx * dax));
                 const double drdj = __builtin_fma(dz, djz, __builtin_fma(dy, djy, dx * djx));
@@ -304,6 +339,7 @@ __device__ __forceinline__ void body_fp64(const double *p, const double *q, doub
 template <int ORDER, bool POT>
 __device__ __forceinline__ void body_split(const double *p, const double *q, float eps2, bool self, int mask_r2, float *A)
 {
+    static_assert(ORDER != 1 && ORDER != 9, "no split-precision body for hi + lo positions");
     const float dx = (float)(q[0] - p[0]), dy = (float)(q[1] - p[1]), dz = (float)(q[2] - p[2]);
     const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, eps2)));
     const float rinv = __builtin_amdgcn_rsqf(r2);
@@ -383,7 +419,7 @@ __global__ __launch_bounds__(NT) void tile_kernel(TileArgs a)
     constexpr int L = lds_words(ORDER);
     constexpr int NC = n_acc(ORDER, POT);
     // i words actually used: pos (+ vel, acc, jrk); mass not needed
-    constexpr int QI = ORDER == 0 ? 3 : Q;
+    constexpr int QI = ORDER == 0 ? 3 : Q;  // order 1: posH, mass (unused), posL
     __shared__ double sj[TJ * L];
 
     const int tid = threadIdx.x;
@@ -397,7 +433,7 @@ __global__ __launch_bounds__(NT) void tile_kernel(TileArgs a)
         ii[k] = iblock * NT * NI + k * NT + tid;
         const int idx = ii[k] < a.nact ? ii[k] : a.nact - 1;
 #pragma unroll
-        for (int w = 0; w < QI; w++) pi[k][w] = a.src[(size_t)w * npad + idx];
+        for (int w = 0; w < QI; w++) pi[k][w] = a.src[(size_t)src_word(ORDER, w) * npad + idx];
     }
     double A[NI][NC];
 #pragma unroll
@@ -411,7 +447,7 @@ __global__ __launch_bounds__(NT) void tile_kernel(TileArgs a)
     double jr[Q];
     if (t0 < t1) {
 #pragma unroll
-        for (int w = 0; w < Q; w++) jr[w] = a.src[(size_t)w * npad + t0 * TJ + tid];
+        for (int w = 0; w < Q; w++) jr[w] = a.src[(size_t)src_word(ORDER, w) * npad + t0 * TJ + tid];
     }
     const float eps2f = (float)a.eps2;
     // a wave whose lanes are all past the i range only helps stage tiles
@@ -424,7 +460,7 @@ __global__ __launch_bounds__(NT) void tile_kernel(TileArgs a)
         __syncthreads();
         if (t + 1 < t1) {
 #pragma unroll
-            for (int w = 0; w < Q; w++) jr[w] = a.src[(size_t)w * npad + (t + 1) * TJ + tid];
+            for (int w = 0; w < Q; w++) jr[w] = a.src[(size_t)src_word(ORDER, w) * npad + (t + 1) * TJ + tid];
         }
         if (!busy) continue;
         const int jbase = t * TJ;
@@ -472,7 +508,7 @@ __global__ __launch_bounds__(NT) void tile_kernel(TileArgs a)
         }
     }
 
-    constexpr int F = ORDER == 0 ? 1 : f_words(ORDER);
+    constexpr int F = ORDER <= 1 ? 1 : f_words(ORDER);
     if (a.nslice == 1) {
 #pragma unroll
         for (int k = 0; k < NI; k++) {

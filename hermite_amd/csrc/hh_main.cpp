@@ -16,8 +16,10 @@
 //   variant fp64|split|exact   force kernel (default fp64)
 //   device  <ordinal>          HIP device (default 0)
 //   dtmin <dt> | dtmininv <1/dt>   lower clamp on the timestep (default none)
-// The order comes from the "order" tag (4, 6 or 8); the reference binary
-// asserts it against the order it was compiled for.
+// The order comes from the "order" tag (4, 6 or 8; "8dd" selects the
+// double-double 8th-order build, which the reference selects at compile
+// time with -DEIGHTHDD and whose parser reads "8dd" as 8); the reference
+// binary asserts the order against the one it was compiled for.
 //
 //   hermite_run < inp.6th          run
 //   hermite_run --params < inp     parse and print the parameters only
@@ -42,6 +44,7 @@ namespace {
 struct Params {
     std::string snapin, snapout_base, dumpin, dumpout_base;
     int order = 0, snapid = 0, dumpid = 0;
+    bool dd = false;  // "order 8dd" 
     double tend, eta, eta_s, dtmax, eps, kt_for_eps;
     double log_interval, snap_interval, dump_interval;
     std::string variant = "fp64";
@@ -71,7 +74,12 @@ struct Params {
             else if (tag == "snapout_base") ss >> snapout_base;
             else if (tag == "dumpin") ss >> dumpin;
             else if (tag == "dumpout_base") ss >> dumpout_base;
-            else if (tag == "order") ss >> order;
+            else if (tag == "order") {
+                std::string v;
+                ss >> v;
+                order = std::atoi(v.c_str());
+                dd = v == "8dd";
+            }
             else if (tag == "snapid") ss >> snapid;
             else if (tag == "dumpid") ss >> dumpid;
             else if (tag == "tend") ss >> tend;
@@ -124,6 +132,7 @@ struct Params {
         if (!(eps >= 0.0 || kt_for_eps >= 0.0)) return "eps >= 0 or kt_for_eps >= 0";
         if (variant != "fp64" && variant != "split" && variant != "exact") return "variant must be fp64, split or exact";
         if (!(dtmin >= 0.0)) return "dtmin >= 0";
+        if (dd && variant == "split") return "order 8dd has no split variant";
         return "";
     }
 };
@@ -160,7 +169,7 @@ bool format_name(const std::string &pat, int id, std::string &out)
 }
 
 int variant_code(const std::string &v) { return v == "split" ? HH_SPLIT : v == "exact" ? HH_EXACT : HH_FP64; }
-int pwords(int o) { return o == 4 ? 24 : o == 6 ? 28 : 36; }
+int pwords(int o) { return o == 4 ? 24 : o == 6 ? 28 : o == 8 ? 36 : 40; }
 int flops(int o) { return o == 4 ? 60 : o == 6 ? 97 : 144; }
 
 // Profile of the reference (SRC/nbodysystem.h:57-194) over the driver's
@@ -242,7 +251,7 @@ int main(int argc, char **argv)
     const std::string bad = prm.check();
     if (!bad.empty()) return die(nullptr, "parameter check failed: " + bad);
 
-    const int order = prm.order;
+    const int order = prm.dd ? HH_ORDER_8DD : prm.order;
     int snapid = prm.snapid, dumpid = prm.dumpid;
     const double eps2 = prm.eps * prm.eps;
     hhd_sys *s = nullptr;

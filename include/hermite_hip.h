@@ -54,6 +54,14 @@ enum {
                      bit-identical to the reference CPU build */
 };
 
+/* Order code of the double-double 8th-order build (EIGHTHDD,
+ * SRC/compile.cpp:47-48, SRC/hermite8dd.h): positions held as hi + lo
+ * doubles (Particle of 40 doubles: id mass tlast dt posH[3] posL[3] vel[3]
+ * acc[3] ... d7a[3] pad[3], SRC/hermite8dd.h:44-78), relative positions
+ * (posHj - posHi) + (posLj - posLi), the 8th-order Force.  Variants HH_FP64
+ * and HH_EXACT; the GPU-resident driver (hhr_*) does not take it. */
+#define HH_ORDER_8DD 9
+
 /* Status codes. */
 enum {
     HH_OK = 0,
@@ -64,15 +72,18 @@ enum {
 };
 
 /* Doubles per record at the boundary and on the device. */
-int hh_particle_words(int order);  /* 24 / 28 / 36  (reference Particle)   */
-int hh_force_words(int order);     /* 6 / 9 / 12    (reference Force)      */
-int hh_jp_words(int order);        /* 14 / 20 / 26  (TITAN GParticle,
-                                      SRC/hermite6-titan.h:20-29)          */
-int hh_pred_words(int order);      /* 7 / 10 / 13   (TITAN GPredictor,
-                                      SRC/hermite6-titan.h:31-36)          */
+int hh_particle_words(int order);  /* 24 / 28 / 36 / 40 (reference Particle) */
+int hh_force_words(int order);     /* 6 / 9 / 12 / 12  (reference Force)      */
+int hh_jp_words(int order);        /* 14 / 20 / 26 / 29 (TITAN GParticle,
+                                      SRC/hermite6-titan.h:20-29; the
+                                      double-double record appends posL) */
+int hh_pred_words(int order);      /* 7 / 10 / 13 / 16 (TITAN GPredictor,
+                                      SRC/hermite6-titan.h:31-36; posL last
+                                      for the double-double build)        */
 
-/* Gravity(int nbody): SRC/hermite6-titan.h:55-62.  order is 4, 6 or 8;
- * variant one of HH_FP64, HH_SPLIT, HH_EXACT; device a HIP ordinal. */
+/* Gravity(int nbody): SRC/hermite6-titan.h:55-62.  order is 4, 6, 8 or
+ * HH_ORDER_8DD; variant one of HH_FP64, HH_SPLIT, HH_EXACT (not HH_SPLIT
+ * for HH_ORDER_8DD); device a HIP ordinal. */
 int hh_create(int order, int nbody, int variant, int device, hh_ctx **out);
 
 /* ~Gravity(): SRC/hermite6-titan.h:63-68. */

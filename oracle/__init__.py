@@ -22,10 +22,11 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF_DIR = os.path.join(HERE, "_ref")
 
-PSTRIDE = {4: 24, 6: 28, 8: 36}
-QSTRIDE = {4: 7, 6: 10, 8: 13}
-FSTRIDE = {4: 6, 6: 9, 8: 12}
-FLOPS = {4: 60, 6: 97, 8: 144}
+# order 9: the double-double 8th-order build (SRC/hermite8dd.h)
+PSTRIDE = {4: 24, 6: 28, 8: 36, 9: 40}
+QSTRIDE = {4: 7, 6: 10, 8: 13, 9: 17}
+FSTRIDE = {4: 6, 6: 9, 8: 12, 9: 12}
+FLOPS = {4: 60, 6: 97, 8: 144, 9: 144}
 
 # word offsets inside a Particle record
 W_ID, W_MASS, W_TLAST, W_DT, W_POS, W_VEL, W_ACC, W_JRK, W_SNP, W_CRK = 0, 1, 2, 3, 4, 7, 10, 13, 16, 19
@@ -105,7 +106,9 @@ def make_particles(order: int, mass, pos, vel, ids=None, tlast: float = 0.0) -> 
     P[:, W_MASS] = mass
     P[:, W_TLAST] = tlast
     P[:, W_POS:W_POS + 3] = pos
-    P[:, W_VEL:W_VEL + 3] = vel
+    # double-double (order 9): posH = pos, posL = 0, vel after posL
+    # (SRC/hermite8dd.h:64-78)
+    P[:, (10 if order == 9 else W_VEL):(10 if order == 9 else W_VEL) + 3] = vel
     return P
 
 
@@ -321,18 +324,19 @@ class Ref:
         L.ref_pow_one_nth_quant.restype = C.c_double
         L.ref_pow_one_nth_quant.argtypes = [C.c_int, C.c_double]
         L.ref_invfordt.argtypes = [C.c_double, _dp, _dp]
-        L.ref_sys_new.restype = C.c_void_p
-        L.ref_sys_new.argtypes = [C.c_int, _dp] + [C.c_double] * 6
-        for name in ("ref_sys_free", "ref_sys_init_step", "ref_sys_integrate_one_dtmax"):
-            getattr(L, name).argtypes = [C.c_void_p]
-        for name in ("ref_sys_energy", "ref_sys_init_energy", "ref_sys_tsys"):
-            getattr(L, name).restype = C.c_double
-            getattr(L, name).argtypes = [C.c_void_p]
-        for name in ("ref_sys_num_step", "ref_sys_num_bstep"):
-            getattr(L, name).restype = C.c_long
-            getattr(L, name).argtypes = [C.c_void_p]
-        L.ref_sys_get_ptcl.argtypes = [C.c_void_p, _dp]
-        L.ref_sys_get_pot.argtypes = [C.c_void_p, _dp]
+        if hasattr(L, "ref_sys_new"):  # the double-double build has no driver
+            L.ref_sys_new.restype = C.c_void_p
+            L.ref_sys_new.argtypes = [C.c_int, _dp] + [C.c_double] * 6
+            for name in ("ref_sys_free", "ref_sys_init_step", "ref_sys_integrate_one_dtmax"):
+                getattr(L, name).argtypes = [C.c_void_p]
+            for name in ("ref_sys_energy", "ref_sys_init_energy", "ref_sys_tsys"):
+                getattr(L, name).restype = C.c_double
+                getattr(L, name).argtypes = [C.c_void_p]
+            for name in ("ref_sys_num_step", "ref_sys_num_bstep"):
+                getattr(L, name).restype = C.c_long
+                getattr(L, name).argtypes = [C.c_void_p]
+            L.ref_sys_get_ptcl.argtypes = [C.c_void_p, _dp]
+            L.ref_sys_get_pot.argtypes = [C.c_void_p, _dp]
         assert L.ref_particle_bytes() == 8 * PSTRIDE[order]
 
     # Gravity interface -------------------------------------------------

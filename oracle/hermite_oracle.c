@@ -24,6 +24,16 @@
  *               SRC/hermite6-titan.h:31-36, SRC/hermite8-titan.h:33-39).
  *   Force     : acc 0-2, jrk 3-5, snp 6-8, crk 9-11; stride 6 / 9 / 12
  *               (SRC/hermite4.h:4-14, SRC/hermite6.h:4-17, SRC/hermite8.h:8-24).
+ *
+ * Order code 9 is the double-double 8th-order build (EIGHTHDD,
+ * SRC/compile.cpp:47-48, SRC/hermite8dd.h), positions held as hi + lo:
+ *   Particle  : id 0, mass 1, tlast 2, dt 3, posH 4-6, posL 7-9, vel 10-12,
+ *               acc 13-15, jrk 16-18, snp 19-21, crk 22-24, d4a 25-27,
+ *               d5a 28-30, d6a 31-33, d7a 34-36, pad 37-39 (40 words = 320 B,
+ *               SRC/hermite8dd.h:44-78)
+ *   Predictor : posH 0-2, mass 3, posL 4-6, id 7, vel 8-10, acc 11-13,
+ *               jrk 14-16 (17 words, SRC/hermite8dd.h:283-312)
+ *   Force     : as order 8.
  */
 #include <math.h>
 #include <stdint.h>
@@ -34,9 +44,14 @@ enum { W_ID = 0, W_MASS = 1, W_TLAST = 2, W_DT = 3, W_POS = 4, W_VEL = 7,
        W_ACC = 10, W_JRK = 13, W_SNP = 16, W_CRK = 19, W_D4A = 22, W_D5A = 25,
        W_D6A = 28, W_D7A = 31 };
 
-int orc_pstride(int order) { return order == 4 ? 24 : order == 6 ? 28 : 36; }
-int orc_predstride(int order) { return order == 4 ? 7 : order == 6 ? 10 : 13; }
+/* double-double 8th order: word offsets of its Particle record */
+enum { D_POSH = 4, D_POSL = 7, D_VEL = 10, D_ACC = 13, D_JRK = 16, D_SNP = 19, D_CRK = 22,
+       D_D4A = 25, D_D5A = 28, D_D6A = 31, D_D7A = 34 };
+
+int orc_pstride(int order) { return order == 4 ? 24 : order == 6 ? 28 : order == 8 ? 36 : 40; }
+int orc_predstride(int order) { return order == 4 ? 7 : order == 6 ? 10 : order == 8 ? 13 : 17; }
 int orc_fstride(int order) { return order == 4 ? 6 : order == 6 ? 9 : 12; }
+static int vel_word(int order) { return order == 9 ? D_VEL : W_VEL; }
 
 /* dvec3 dot product (x*v.x + y*v.y + z*v.z), SRC/vector3.h:121-123 */
 static inline double dot3(const double *a, const double *b)
@@ -108,11 +123,37 @@ static void predict_one(int order, const double *p, double tsys, double *q)
     q[3] = p[W_MASS];
 }
 
+/* GPredictor of the double-double build (SRC/hermite8dd.h:291-311): posH
+ * kept, the Taylor displacement added to posL. */
+static void predict_one_dd(const double *p, double tsys, double *q)
+{
+    const double dt = tsys - p[W_TLAST];
+    const double dt2 = (1. / 2.) * dt;
+    const double dt3 = (1. / 3.) * dt;
+    const double dt4 = (1. / 4.) * dt;
+    const double dt5 = (1. / 5.) * dt;
+    const double dt6 = (1. / 6.) * dt;
+    const double dt7 = (1. / 7.) * dt;
+    for (int k = 0; k < 3; k++) {
+        const double xl = p[D_POSL + k], v = p[D_VEL + k], a = p[D_ACC + k], j = p[D_JRK + k];
+        const double sn = p[D_SNP + k], c = p[D_CRK + k], d4 = p[D_D4A + k], d5 = p[D_D5A + k];
+        q[0 + k] = p[D_POSH + k];
+        q[4 + k] = xl + dt * (v + dt2 * (a + dt3 * (j + dt4 * (sn + dt5 * (c + dt6 * (d4 + dt7 * (d5)))))));
+        q[8 + k] = v + dt * (a + dt2 * (j + dt3 * (sn + dt4 * (c + dt5 * (d4 + dt6 * (d5))))));
+        q[11 + k] = a + dt * (j + dt2 * (sn + dt3 * (c + dt4 * (d4 + dt5 * (d5)))));
+        q[14 + k] = j + dt * (sn + dt2 * (c + dt3 * (d4 + dt4 * (d5))));
+    }
+    q[3] = p[W_MASS];
+    q[7] = p[W_ID];
+}
+
 void orc_predict(int order, int n, const double *ptcl, double tsys, double *pred)
 {
     const int ps = orc_pstride(order), qs = orc_predstride(order);
-    for (int i = 0; i < n; i++)
-        predict_one(order, ptcl + (size_t)i * ps, tsys, pred + (size_t)i * qs);
+    for (int i = 0; i < n; i++) {
+        if (order == 9) predict_one_dd(ptcl + (size_t)i * ps, tsys, pred + (size_t)i * qs);
+        else predict_one(order, ptcl + (size_t)i * ps, tsys, pred + (size_t)i * qs);
+    }
 }
 
 /* ------------------------------------------------------------------ */
@@ -179,10 +220,59 @@ static void force_one(int order, int nj, const double *pred, int i, double eps2,
     }
 }
 
+/* Force of the double-double build (SRC/hermite8dd.h:339-389): relative
+ * position (posHj - posHi) + (posLj - posLi), pairs with r2 == eps2 skipped
+ * (the self-pair), otherwise the 8th-order chain of SRC/hermite8.h. */
+static void force_one_dd(int nj, const double *pred, int i, double eps2, double *f)
+{
+    const double *pi = pred + (size_t)i * 17;
+    double acc[3] = {0, 0, 0}, jrk[3] = {0, 0, 0}, snp[3] = {0, 0, 0}, crk[3] = {0, 0, 0};
+    for (int j = 0; j < nj; j++) {
+        const double *pj = pred + (size_t)j * 17;
+        double dr[3], dv[3], da[3], dj[3];
+        for (int k = 0; k < 3; k++) {
+            dr[k] = (pj[0 + k] - pi[0 + k]) + (pj[4 + k] - pi[4 + k]);
+            dv[k] = pj[8 + k] - pi[8 + k];
+            da[k] = pj[11 + k] - pi[11 + k];
+            dj[k] = pj[14 + k] - pi[14 + k];
+        }
+        const double r2 = eps2 + dot3(dr, dr);
+        if (r2 == eps2) continue;
+        const double drdv = dot3(dr, dv);
+        const double dvdv = dot3(dv, dv);
+        const double drda = dot3(dr, da);
+        const double dvda = dot3(dv, da);
+        const double drdj = dot3(dr, dj);
+        const double ri2 = 1.0 / r2;
+        const double mri3 = pj[3] * ri2 * sqrt(ri2);
+        const double alpha = drdv * ri2;
+        const double beta = (dvdv + drda) * ri2 + alpha * alpha;
+        const double gamma = (3.0 * dvda + drdj) * ri2 + alpha * (3.0 * beta - 4.0 * alpha * alpha);
+        for (int k = 0; k < 3; k++) {
+            acc[k] = acc[k] + mri3 * dr[k];
+            const double tmp1 = dv[k] + (-3.0 * alpha) * dr[k];
+            jrk[k] = jrk[k] + mri3 * tmp1;
+            const double tmp2 = da[k] + (-6.0 * alpha) * tmp1 + (-3.0 * beta) * dr[k];
+            snp[k] = snp[k] + mri3 * tmp2;
+            const double tmp3 = dj[k] + (-9.0 * alpha) * tmp2 + (-9.0 * beta) * tmp1 + (-3.0 * gamma) * dr[k];
+            crk[k] = crk[k] + mri3 * tmp3;
+        }
+    }
+    for (int k = 0; k < 3; k++) {
+        f[0 + k] = acc[k];
+        f[3 + k] = jrk[k];
+        f[6 + k] = snp[k];
+        f[9 + k] = crk[k];
+    }
+}
+
 void orc_force(int order, int nj, const double *pred, int nact, double eps2, double *force)
 {
     const int fs = orc_fstride(order);
-    for (int i = 0; i < nact; i++) force_one(order, nj, pred, i, eps2, force + (size_t)i * fs);
+    for (int i = 0; i < nact; i++) {
+        if (order == 9) force_one_dd(nj, pred, i, eps2, force + (size_t)i * fs);
+        else force_one(order, nj, pred, i, eps2, force + (size_t)i * fs);
+    }
 }
 
 /* Same force on an arbitrary i-list (indices into pred), for tests that
@@ -191,8 +281,10 @@ void orc_force_list(int order, int nj, const double *pred, int ni, const int *il
                     double eps2, double *force)
 {
     const int fs = orc_fstride(order);
-    for (int ii = 0; ii < ni; ii++)
-        force_one(order, nj, pred, ilist[ii], eps2, force + (size_t)ii * fs);
+    for (int ii = 0; ii < ni; ii++) {
+        if (order == 9) force_one_dd(nj, pred, ilist[ii], eps2, force + (size_t)ii * fs);
+        else force_one(order, nj, pred, ilist[ii], eps2, force + (size_t)ii * fs);
+    }
 }
 
 /* Potential over corrected positions: order 4 excludes j by r2 > eps2
@@ -207,7 +299,10 @@ void orc_potential(int order, int n, const double *ptcl, double eps2, double *po
         for (int j = 0; j < n; j++) {
             const double *pj = ptcl + (size_t)j * ps;
             double dr[3];
-            for (int k = 0; k < 3; k++) dr[k] = pj[W_POS + k] - pi[W_POS + k];
+            /* double-double: (posHj - posHi) + (posLj - posLi), SRC/hermite8dd.h:391-413 */
+            for (int k = 0; k < 3; k++)
+                dr[k] = order == 9 ? (pj[D_POSH + k] - pi[D_POSH + k]) + (pj[D_POSL + k] - pi[D_POSL + k])
+                                   : pj[W_POS + k] - pi[W_POS + k];
             const double r2 = eps2 + dot3(dr, dr);
             double mj;
             if (order == 4) mj = (r2 > eps2) ? pj[W_MASS] : 0.0;
@@ -225,8 +320,27 @@ static inline double norm2(const double *a) { return dot3(a, a); }
 
 /* aarseth_step_quant: SRC/hermite4.h:78-94, SRC/hermite6.h:87-108,
  * SRC/hermite8.h:112-135.  p points to a Particle record. */
+/* The DD record without posL is the 8th-order record: view and back. */
+static void dd_to_8(const double *d, double *e)
+{
+    memcpy(e, d, 4 * 8);
+    memcpy(e + W_POS, d + D_POSH, 3 * 8);
+    memcpy(e + W_VEL, d + D_VEL, 27 * 8);
+}
+static void dd_from_8(const double *e, double *d)
+{
+    memcpy(d, e, 4 * 8);
+    memcpy(d + D_POSH, e + W_POS, 3 * 8);
+    memcpy(d + D_VEL, e + W_VEL, 27 * 8);
+}
+
 double orc_aarseth_step_quant(int order, const double *p, double etapow)
 {
+    if (order == 9) {
+        double e[36];
+        dd_to_8(p, e);
+        return orc_aarseth_step_quant(8, e, etapow);
+    }
     const double s0 = norm2(p + W_ACC), s1 = norm2(p + W_JRK), s2 = norm2(p + W_SNP);
     if (order == 4) {
         const double s3 = norm2(p + W_CRK);
@@ -249,6 +363,15 @@ double orc_aarseth_step_quant(int order, const double *p, double etapow)
 /* assign_force: SRC/hermite4.h:67-70, SRC/hermite6.h:75-79, SRC/hermite8.h:91-96 */
 void orc_assign_force(int order, double *p, const double *f)
 {
+    if (order == 9) {
+        for (int k = 0; k < 3; k++) {
+            p[D_ACC + k] = f[0 + k];
+            p[D_JRK + k] = f[3 + k];
+            p[D_SNP + k] = f[6 + k];
+            p[D_CRK + k] = f[9 + k];
+        }
+        return;
+    }
     for (int k = 0; k < 3; k++) {
         p[W_ACC + k] = f[0 + k];
         p[W_JRK + k] = f[3 + k];
@@ -260,6 +383,13 @@ void orc_assign_force(int order, double *p, const double *f)
 /* init_dt: SRC/hermite4.h:72-76, SRC/hermite6.h:81-85, SRC/hermite8.h:98-110 */
 void orc_init_dt(int order, double *p, double eta_s, double dtmax)
 {
+    if (order == 9) {  /* as 8th order, SRC/hermite8dd.h:134-147 */
+        double e[36];
+        dd_to_8(p, e);
+        orc_init_dt(8, e, eta_s, dtmax);
+        p[W_DT] = e[W_DT];
+        return;
+    }
     double dtnat;
     if (order == 4) {
         dtnat = eta_s * sqrt(norm2(p + W_ACC) / norm2(p + W_JRK));
@@ -280,8 +410,81 @@ void orc_init_dt(int order, double *p, double eta_s, double dtmax)
 /* Hermite corrector with derivative reconstruction and the quantised
  * Aarseth step: SRC/hermite4.h:118-153, SRC/hermite6.h:136-184,
  * SRC/hermite8.h:166-222. */
+/* dd_accumulate (SRC/hermite8dd.h:4-24, the second form): adds dx to the
+ * pair xh + xl and renormalises. */
+static void dd_accumulate(double *xh, double *xl, double dx)
+{
+    const double yh = *xh + dx;
+    const double res = dx - (yh - *xh);
+    const double yl = *xl + res;
+    const double zh = yh + yl;
+    const double zl = yl - (zh - yh);
+    *xh = zh;
+    *xl = zl;
+}
+
+/* Corrector of the double-double build (SRC/hermite8dd.h:187-238): the
+ * 8th-order corrector with h = 0.5 dt, hinv = 2 / dt and the position
+ * increment added to posH + posL by dd_accumulate. */
+static void correct_dd(double *p, const double *f, double etapow, double dtlim)
+{
+    const double dt = p[W_DT];
+    const double h = 0.5 * dt;
+    const double hinv = 2.0 / dt;
+    double *vel = p + D_VEL, *acc = p + D_ACC, *jrk = p + D_JRK, *snp = p + D_SNP, *crk = p + D_CRK;
+    double *d4a = p + D_D4A, *d5a = p + D_D5A, *d6a = p + D_D6A, *d7a = p + D_D7A;
+    const double *facc = f, *fjrk = f + 3, *fsnp = f + 6, *fcrk = f + 9;
+    double Ap[3], Am[3], Jp[3], Jm[3], Sp[3], Sm[3], Cp[3], Cm[3];
+    for (int k = 0; k < 3; k++) {
+        Ap[k] = facc[k] + acc[k];
+        Am[k] = facc[k] - acc[k];
+        Jp[k] = (fjrk[k] + jrk[k]) * h;
+        Jm[k] = (fjrk[k] - jrk[k]) * h;
+        Sp[k] = (fsnp[k] + snp[k]) * (h * h);
+        Sm[k] = (fsnp[k] - snp[k]) * (h * h);
+        Cp[k] = (fcrk[k] + crk[k]) * (h * h * h);
+        Cm[k] = (fcrk[k] - crk[k]) * (h * h * h);
+    }
+    for (int k = 0; k < 3; k++) {
+        const double vel1 = vel[k] + h * (Ap[k] + (-3. / 7.) * Jm[k] + (2. / 21) * Sp[k] - (1. / 105.) * Cm[k]);
+        const double dpos = h * ((vel[k] + vel1) + h * ((-3. / 7.) * Am[k] + (2. / 21) * Jp[k] - (1. / 105.) * Sm[k]));
+        dd_accumulate(p + D_POSH + k, p + D_POSL + k, dpos);
+        vel[k] = vel1;
+        acc[k] = facc[k];
+        jrk[k] = fjrk[k];
+        snp[k] = fsnp[k];
+        crk[k] = fcrk[k];
+    }
+    p[W_TLAST] += dt;
+    const double hinv2 = hinv * hinv;
+    const double hinv3 = hinv2 * hinv;
+    const double hinv4 = hinv2 * hinv2;
+    const double hinv5 = hinv2 * hinv3;
+    const double hinv6 = hinv3 * hinv3;
+    const double hinv7 = hinv3 * hinv4;
+    const double third = 1.0 / 3.;
+    for (int k = 0; k < 3; k++) {
+        d4a[k] = (hinv4 * 24. / 32.) * (-5. * Jm[k] + 5. * Sp[k] - Cm[k]);
+        d5a[k] = (hinv5 * 120. / 32.) * (21. * Am[k] - 21. * Jp[k] + 8. * Sm[k] - Cp[k]);
+        d6a[k] = (hinv6 * 720. / 32.) * (Jm[k] - Sp[k] + Cm[k] * third);
+        d7a[k] = (hinv7 * 5040. / 32.) * (-5. * Am[k] + 5. * Jp[k] - 2. * Sm[k] + Cp[k] * third);
+    }
+    const double h2 = 0.5 * h, h3 = (1. / 3.) * h;
+    for (int k = 0; k < 3; k++) {
+        d4a[k] = d4a[k] + h * (d5a[k] + h2 * (d6a[k] + h3 * d7a[k]));
+        d5a[k] = d5a[k] + h * (d6a[k] + h2 * d7a[k]);
+        d6a[k] = d6a[k] + h * d7a[k];
+    }
+    const double dtq = orc_aarseth_step_quant(9, p, etapow);
+    p[W_DT] = dtq < dtlim ? dtq : dtlim;
+}
+
 void orc_correct(int order, double *p, const double *f, double etapow, double dtlim)
 {
+    if (order == 9) {
+        correct_dd(p, f, etapow, dtlim);
+        return;
+    }
     double h, hinv;
     orc_invfordt(p[W_DT], &h, &hinv);
     double *pos = p + W_POS, *vel = p + W_VEL, *acc = p + W_ACC, *jrk = p + W_JRK;
@@ -483,7 +686,7 @@ orc_sys *orc_sys_new(int order, int n, const double *ptcl_in, double tsys, doubl
     s->eta_s = eta_s;
     s->dtmax = dtmax;
     s->tsys = tsys;
-    s->etapow = pow(eta, (double)(2 * (order - 3)));
+    s->etapow = pow(eta, (double)(2 * ((order == 9 ? 8 : order) - 3)));
     s->ptcl = (double *)malloc((size_t)n * s->ps * 8);
     s->force = (double *)calloc((size_t)n * s->fs, 8);
     s->pred = (double *)calloc((size_t)n * s->qs, 8);
@@ -508,7 +711,7 @@ double orc_sys_energy(orc_sys *s)
     double ke = 0.0, pe = 0.0;
     for (int i = 0; i < s->n; i++) {
         const double *p = s->ptcl + (size_t)i * s->ps;
-        ke += p[W_MASS] * norm2(p + W_VEL);
+        ke += p[W_MASS] * norm2(p + vel_word(s->order));
         pe += p[W_MASS] * s->pot[i];
     }
     ke *= 0.5;

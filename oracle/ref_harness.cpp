@@ -25,10 +25,18 @@ typedef double v4df __attribute__((vector_size(32)));
 #include "hermite6.h"
 #elif defined EIGHTH
 #include "hermite8.h"
+#elif defined EIGHTHDD
+// The double-double build's physics and generic Gravity only: the
+// reference's driver does not compile for it (SRC/nbodysystem.h:366 tests
+// the misspelt macro EIGTHDD and then reads Particle::pos, which the
+// double-double Particle does not have).
+#include "hermite8dd.h"
 #else
-#error "define FOURTH, SIXTH or EIGHTH"
+#error "define FOURTH, SIXTH, EIGHTH or EIGHTHDD"
 #endif
+#ifndef EIGHTHDD
 #include "nbodysystem.h"
+#endif
 
 namespace {
 Particle *as_particle(void *buf) { return reinterpret_cast<Particle *>(buf); }
@@ -97,6 +105,7 @@ void ref_invfordt(double dt, double *h, double *hinv)
     *hinv = t.hinv;
 }
 
+#ifndef EIGHTHDD
 // ---- Block-step driver (SRC/nbodysystem.h struct NbodySystem) ----
 void *ref_sys_new(int n, const double *recs, double tsys, double eps2, double eta, double eta_s,
                   double dtmax, double tend)
@@ -127,6 +136,7 @@ void ref_sys_get_pot(void *s, double *pot)
     NbodySystem *sy = static_cast<NbodySystem *>(s);
     std::memcpy(pot, sy->potbuf, sizeof(double) * (size_t)sy->nbody);
 }
+#endif
 }
 
 #ifdef REF_MAIN
