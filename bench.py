@@ -15,9 +15,10 @@ predict_all, force on the active list, corrector and timestep on the
 device).  One step is one dt_max interval of integration (the reference's
 integrate_one_dtmax, SRC/nbodysystem.h:904-944): the timed region is
 steps x dt_max time units from t = 0, started and ended by a stream
-synchronisation inside hhr_run, with the block steps launched directly on
-one stream (--graph: replayed from a hipGraph).  Warmup intervals run on a
-separate context that is discarded, so the timed run starts from t = 0.
+synchronisation inside hhr_run, with the block steps replayed from a
+hipGraph of 16 steps (--direct: launched directly on one stream).  Warmup
+intervals run on a separate context that is discarded, so the timed run
+starts from t = 0.
 
 value = interactions/s of the plain FP64 kernel over the timed run
 (sum of nact * N over block steps / wall), with |dE/E| at its end.  A second,
@@ -269,7 +270,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1, help="dt_max intervals on a discarded context")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-tend", type=float, default=0.0, help="time span of the CPU baseline run (default dt_max)")
-    ap.add_argument("--graph", action="store_true", help="replay the timed steps from a hipGraph")
+    ap.add_argument("--direct", action="store_true",
+                    help="launch the timed steps directly on the stream instead of replaying a hipGraph")
     ap.add_argument("--single-pass", action="store_true",
                     help="time the event-timed pass itself instead of a separate untimed run (default for the "
                          "long workloads c3 and c5; the events add about 10 us per block step)")
@@ -293,6 +295,7 @@ def main():
         log("calibration copies done (5 x 256 MiB device-to-device)")
         del src, dst
     w = W.workload(args.workload)
+    graph = not args.direct
     steps = args.steps or int(round(w.tend / w.dtmax))
     tend = steps * w.dtmax
     cpu_tend = args.cpu_tend or w.dtmax
@@ -300,11 +303,11 @@ def main():
     states = initial_states(w)
     if args.warmup > 0:
         log(f"warmup: {args.warmup} interval(s) on a discarded context")
-        integrate(w, "fp64", states, min(args.warmup, steps) * w.dtmax, graph=args.graph)
+        integrate(w, "fp64", states, min(args.warmup, steps) * w.dtmax, graph=graph)
     single = args.single_pass or w.name in ("c3", "c5")
     chunks = 16 if w.name in ("c3", "c5") else 1
     log(f"timed: FP64 resident integration t=0->{tend:g}" + (" (event-timed)" if single else ""))
-    f = integrate(w, "fp64", states, tend, timing=single, graph=args.graph, chunks=chunks)
+    f = integrate(w, "fp64", states, tend, timing=single, graph=graph, chunks=chunks)
     if f["error"]:
         raise SystemExit(f"FP64 integration failed: {f['error']}")
     value = f["interactions"] / f["wall_s"]
@@ -350,7 +353,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded Plummer spheres)",
-        "config": {"workload": workload_text(w, tend, args.graph), "name": w.name, "order": w.order,
+        "config": {"workload": workload_text(w, tend, graph and not single), "name": w.name, "order": w.order,
                    "n": n_j, "clusters": len(w.clusters), "kernel": "fp64"},
         "gflops": value * H.FLOPS[w.order] * 1e-9,
         "dE_E": f["dE_E"],

@@ -52,7 +52,7 @@ enum { DD_POSH = 4, DD_POSL = 7, DD_VEL = 10, DD_ACC = 13, DD_JRK = 16, DD_SNP =
        DD_D4A = 25, DD_D5A = 28, DD_D6A = 31, DD_D7A = 34 };
 constexpr int pwords(int o) { return o == 4 ? 24 : o == 6 ? 28 : o == 8 ? 36 : 40; }
 constexpr int fwords(int o) { return o == 4 ? 6 : o == 6 ? 9 : 12; }
-constexpr int vel_word(int o) { return o == 9 ? DD_VEL : VEL; }
+constexpr int vel_word(int o) { return o == 9 ? int(DD_VEL) : int(VEL); }
 constexpr bool valid_order(int o) { return o == 4 || o == 6 || o == 8 || o == 9; }
 
 // pow_one_nth_quant<N>: 2^floor(log2(x)/N) from the exponent bits

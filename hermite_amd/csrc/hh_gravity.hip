@@ -651,7 +651,7 @@ struct hhr_ctx {
     int force_grid = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    double *rp = nullptr, *jpr = nullptr, *tn = nullptr, *part = nullptr, *pot = nullptr;
+    double *rp = nullptr, *jpr = nullptr, *ipr = nullptr, *tn = nullptr, *part = nullptr, *pot = nullptr;
     int *ilist = nullptr, *tile_clu = nullptr;
     RClu *clu = nullptr;
     RGlob *glob = nullptr;
@@ -701,6 +701,7 @@ static RArgs r_args(hhr_ctx *c)
     RArgs a;
     a.rp = c->rp;
     a.jpr = c->jpr;
+    a.ipr = c->ipr;
     a.tn = c->tn;
     a.ilist = c->ilist;
     a.tile_clu = c->tile_clu;
@@ -767,6 +768,7 @@ static void r_free(hhr_ctx *c)
     if (c->graph) (void)hipGraphDestroy(c->graph);
     (void)hipFree(c->rp);
     (void)hipFree(c->jpr);
+    (void)hipFree(c->ipr);
     (void)hipFree(c->tn);
     (void)hipFree(c->part);
     (void)hipFree(c->pot);
@@ -913,6 +915,7 @@ int hhr_create(int order, int variant, int device, int nclu, const int *ns, cons
     if ((e = hipEventCreate(&c->ev1)) != hipSuccess) return bail("event", e);
     if ((e = hipMalloc(&c->rp, sizeof(double) * img.size())) != hipSuccess) return bail("records", e);
     if ((e = hipMalloc(&c->jpr, sizeof(double) * (size_t)Q * tot)) != hipSuccess) return bail("predictions", e);
+    if ((e = hipMalloc(&c->ipr, sizeof(double) * (size_t)Q * tot)) != hipSuccess) return bail("active predictions", e);
     if ((e = hipMalloc(&c->tn, sizeof(double) * tot)) != hipSuccess) return bail("tn", e);
     if ((e = hipMalloc(&c->pot, sizeof(double) * tot)) != hipSuccess) return bail("pot", e);
     if ((e = hipMalloc(&c->part, sizeof(double) * c->part_cap)) != hipSuccess) return bail("partials", e);

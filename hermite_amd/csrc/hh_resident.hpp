@@ -84,6 +84,7 @@ struct RGlob {
 struct RArgs {
     double *__restrict__ rp;     // resident records, SoA
     double *__restrict__ jpr;    // predictions, SoA
+    double *__restrict__ ipr;    // predictions of the active particles, SoA at their active-list positions
     double *__restrict__ tn;     // tlast + dt per slot (+inf on padding)
     int *__restrict__ ilist;     // active list, at each cluster's offset
     const int *__restrict__ tile_clu;
@@ -304,8 +305,9 @@ __global__ __launch_bounds__(NT) void r_compact_kernel(RArgs a)
         s_base = tot > 0 ? atomicAdd(&c.acnt[p], tot) : 0;
         if (m < r_inf()) atomicMin(&c.tmin_slot[p ^ 1][tile % R_SLOTS], r_bits(m));
     }
+    double q[Q];
     if (live) {
-        double r[W], q[Q];
+        double r[W];
 #pragma unroll
         for (int w = 0; w < W; w++) r[w] = a.rp[(size_t)w * a.npad + i];
         predict_words<ORDER, EX>(r, tmin, q);
@@ -317,7 +319,11 @@ __global__ __launch_bounds__(NT) void r_compact_kernel(RArgs a)
         int base = s_base;
         for (int w = 0; w < wave; w++) base += wcnt[w];
         const unsigned long long below = lane == 0 ? 0ULL : (mask & ((~0ULL) >> (64 - lane)));
-        a.ilist[c.off + base + __popcll(below)] = i;
+        const int pos = c.off + base + __popcll(below);
+        a.ilist[pos] = i;
+        // the i-side copy the force kernel reads without the ilist indirection
+#pragma unroll
+        for (int w = 0; w < Q; w++) a.ipr[(size_t)w * a.npad + pos] = q[w];
     }
 }
 
@@ -514,7 +520,7 @@ __device__ int r_find_cluster(const RClu *clu, int nclu, int task)
 // task kinds runs the tile tasks faster than a tile-only kernel with fewer
 // registers and twice the waves per SIMD.)
 template <int ORDER, int MODE>
-__global__ __launch_bounds__(NT) void r_force_kernel(RArgs a)
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void r_force_kernel(RArgs a)
 {
     constexpr int Q = pr_words(ORDER);
     constexpr int L = lds_words(ORDER);
@@ -564,9 +570,9 @@ __global__ __launch_bounds__(NT) void r_force_kernel(RArgs a)
 #pragma unroll
             for (int k = 0; k < NI; k++) {
                 ii[k] = grp * GS + k * (64 * wi_n) + wi * 64 + lane;
-                const int gi = a.ilist[c.off + min(ii[k], c.nact - 1)];
+                const int ip = c.off + min(ii[k], c.nact - 1);
 #pragma unroll
-                for (int w = 0; w < Q; w++) pi[k][w] = src[(size_t)w * npad + gi];
+                for (int w = 0; w < Q; w++) pi[k][w] = a.ipr[(size_t)w * npad + ip];
             }
             // a wave whose lanes are all past the active list only helps
             // stage the tiles; its SIMD slots go to the other waves
@@ -663,13 +669,12 @@ __global__ __launch_bounds__(NT) void r_force_kernel(RArgs a)
             }
         } else if constexpr (MODE != EXACT) {  // EXACT always plans mode 1
             // ---- j in lanes, R_GI i-particles per group
-            int gi[R_GI];
             double pi[R_GI][Q];
 #pragma unroll
             for (int g = 0; g < R_GI; g++) {
-                gi[g] = a.ilist[c.off + min(grp * R_GI + g, c.nact - 1)];
+                const int ip = c.off + min(grp * R_GI + g, c.nact - 1);
 #pragma unroll
-                for (int w = 0; w < Q; w++) pi[g][w] = src[(size_t)w * npad + gi[g]];
+                for (int w = 0; w < Q; w++) pi[g][w] = a.ipr[(size_t)w * npad + ip];
             }
             double A[R_GI][NC];
 #pragma unroll
@@ -732,30 +737,29 @@ __device__ int r_find_active(const RClu *clu, int nclu, int pos)
     return lo;
 }
 
-// Sum every active particle's slice partials (RL lanes per particle, fixed
+// Sum every active particle's slice partials (RLX lanes per particle, fixed
 // xor tree) and apply the corrector.  Grid-stride over the active list of
-// all clusters, so the launch shape is fixed (graph-capturable).
-template <int ORDER>
-__global__ __launch_bounds__(NT) void r_reduce_correct_kernel(RArgs a)
+// all clusters, so the launch shape is fixed (graph-capturable).  Short
+// lists take a whole wave per particle (one round of loads per lane), long
+// ones 8 lanes per particle; the order of additions is fixed by the slice
+// count and the active count of the step.
+template <int RLX>
+__device__ __forceinline__ double r_lane_tree(double x)
+{
+#pragma unroll
+    for (int off = RLX / 2; off >= 1; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
+template <int ORDER, int RLX>
+__device__ __forceinline__ void r_reduce_body(const RArgs &a, const int *s_key, bool keys_lds, int nclu, int total,
+                                              int nb, double &tacc, int &kacc)
 {
     constexpr int NC = f_words(ORDER);
-    constexpr int PB = NT / RL;  // particles per workgroup pass
-    __shared__ int s_key[R_CLU_LDS];
-    const int nclu = a.glob->nclu, total = a.glob->total_active;
-    if (blockIdx.x * PB >= total) return;  // whole workgroup idle this step
-    const bool keys_lds = nclu > 1 && nclu <= R_CLU_LDS;
-    if (keys_lds) {
-        for (int k = threadIdx.x; k < nclu; k += NT) s_key[k] = a.clu[k].act0;
-        __syncthreads();
-    }
-    const int l = threadIdx.x % RL;
-    const int nb = a.glob->step & 1;  // the plan has advanced the step: next buffers
-    // each lane keeps the minimum of the new block times it produced for
-    // one cluster and flushes it when the cluster changes
-    double tacc = r_inf();
-    int kacc = -1;
+    constexpr int PB = NT / RLX;  // particles per workgroup pass
+    const int l = threadIdx.x % RLX;
     for (int base = blockIdx.x * PB; base < total; base += gridDim.x * PB) {
-        const int pos = base + threadIdx.x / RL;
+        const int pos = base + threadIdx.x / RLX;
         const bool live = pos < total;
         const int k = !live || nclu == 1 ? 0 : keys_lds ? r_find_lds(s_key, nclu, pos) : r_find_active(a.clu, nclu, pos);
         const RClu &c = a.clu[k];
@@ -766,13 +770,13 @@ __global__ __launch_bounds__(NT) void r_reduce_correct_kernel(RArgs a)
         if (live) {
             const double *part = a.part + c.pofs;
 #pragma unroll 4
-            for (int sl = l; sl < c.nslice; sl += RL) {
+            for (int sl = l; sl < c.nslice; sl += RLX) {
 #pragma unroll
                 for (int q = 0; q < NC; q++) S[q] += part[((size_t)sl * NC + q) * c.ipad + ii];
             }
         }
 #pragma unroll
-        for (int q = 0; q < NC; q++) S[q] = lane_tree8(S[q]);
+        for (int q = 0; q < NC; q++) S[q] = r_lane_tree<RLX>(S[q]);
         if (live && l == 0) {
             const double tn = r_finish<ORDER>(a, c, a.ilist[c.off + ii], S);
             if (kacc != k && kacc >= 0) atomicMin(&a.clu[kacc].tmin_slot[nb][blockIdx.x % R_SLOTS], r_bits(tacc));
@@ -781,6 +785,27 @@ __global__ __launch_bounds__(NT) void r_reduce_correct_kernel(RArgs a)
             tacc = fmin(tacc, tn);
         }
     }
+}
+
+template <int ORDER>
+__global__ __launch_bounds__(NT) void r_reduce_correct_kernel(RArgs a)
+{
+    __shared__ int s_key[R_CLU_LDS];
+    const int nclu = a.glob->nclu, total = a.glob->total_active;
+    const bool wide = (long long)total * 64 <= (long long)gridDim.x * NT;
+    if (blockIdx.x * (wide ? NT / 64 : NT / RL) >= total) return;  // whole workgroup idle this step
+    const bool keys_lds = nclu > 1 && nclu <= R_CLU_LDS;
+    if (keys_lds) {
+        for (int k = threadIdx.x; k < nclu; k += NT) s_key[k] = a.clu[k].act0;
+        __syncthreads();
+    }
+    const int nb = a.glob->step & 1;  // the plan has advanced the step: next buffers
+    // each lane keeps the minimum of the new block times it produced for
+    // one cluster and flushes it when the cluster changes
+    double tacc = r_inf();
+    int kacc = -1;
+    if (wide) r_reduce_body<ORDER, 64>(a, s_key, keys_lds, nclu, total, nb, tacc, kacc);
+    else r_reduce_body<ORDER, RL>(a, s_key, keys_lds, nclu, total, nb, tacc, kacc);
     // one atomic per workgroup when all its lanes worked for one cluster
     __shared__ int s_k[NT / 64];
     __shared__ double s_t[NT / 64];
