@@ -87,10 +87,11 @@ def initial_states(w):
     return out
 
 
-def integrate(w, variant, states, tend, timing=False, graph=False, chunks=1):
+def integrate(w, variant, states, tend, timing=False, graph=False, chunks=1, wall_limit=None):
     """Resident integration t = 0 -> tend, in `chunks` equal time spans
     (each hhr_run call ends with a stream synchronisation; a progress line
-    per chunk)."""
+    per chunk).  With wall_limit (s) the run stops after the first chunk
+    that ends past it."""
     clus = [(P0, eps2, w.eta, w.dtmax, w.dtmin, 0.0) for P0, eps2, _, _ in states]
     r = H.Resident(w.order, variant, clus)
     try:
@@ -103,6 +104,9 @@ def integrate(w, variant, states, tend, timing=False, graph=False, chunks=1):
                 r.run(tend * k / chunks, use_graph=graph and not timing, batch=BATCH)
                 if chunks > 1:
                     log(f"  {variant} t={tend * k / chunks:g}: {time.perf_counter() - t0:.1f} s")
+                if wall_limit is not None and k < chunks and time.perf_counter() - t0 > wall_limit:
+                    err = f"stopped by the benchmark after {wall_limit:g} s of wall time"
+                    break
         except H.HHError as ex:
             err = str(ex)
         wall = time.perf_counter() - t0
@@ -323,9 +327,13 @@ def main():
     achieved = flops_per_launch / (avg_ms * 1e-3) * 1e-12
     split, ap2 = None, None
     if not args.no_split:
+        # 6th and 8th order do not integrate with the FP32 body (step
+        # collapse, or steps pinned at dt_min when there is a clamp): one
+        # interval at most, in small chunks, for at most 30 s
         split_tend = tend if w.order == 4 else w.dtmax
         log(f"split-precision integration t=0->{split_tend:g}")
-        s = integrate(w, "split", states, split_tend, chunks=chunks)
+        s = integrate(w, "split", states, split_tend, chunks=chunks if w.order == 4 else 256,
+                      wall_limit=None if w.order == 4 else 30.0)
         log(f"split: t={s['tsys']:.3e} after {s['num_bstep']} block steps: {s['error'] or 'ok'}")
         split = {"integration_error": s["error"], "tsys_reached": s["tsys"], "block_steps": s["num_bstep"],
                  "interactions_per_s": s["interactions"] / s["wall_s"], "dE_E": s.get("dE_E"),

@@ -648,7 +648,7 @@ int hh_last_geometry(const hh_ctx *c, int *kernel, int *jsplit, int *workgroups)
 
 struct hhr_ctx {
     int order = 0, variant = 0, device = 0, nclu = 0, npad = 0, ntiles = 0, rw = 0;
-    int force_grid = 0;
+    int force_grid = 0, reduce_grid = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double *rp = nullptr, *jpr = nullptr, *ipr = nullptr, *tn = nullptr, *part = nullptr, *pot = nullptr;
@@ -732,7 +732,7 @@ static hipError_t r_launch_step(hhr_ctx *c, int s)
     if ((e = r_mark(c, 4 * s + 1)) != hipSuccess) return e;
     hipLaunchKernelGGL((r_force_kernel<ORDER, MODE>), dim3(c->force_grid), dim3(NT), 0, c->stream, a);
     if ((e = r_mark(c, 4 * s + 2)) != hipSuccess) return e;
-    hipLaunchKernelGGL((r_reduce_correct_kernel<ORDER>), dim3(c->force_grid), dim3(NT), 0, c->stream, a);
+    hipLaunchKernelGGL((r_reduce_correct_kernel<ORDER>), dim3(c->reduce_grid), dim3(NT), 0, c->stream, a);
     if ((e = r_mark(c, 4 * s + 3)) != hipSuccess) return e;
     return hipGetLastError();
 }
@@ -929,7 +929,11 @@ int hhr_create(int order, int variant, int device, int nclu, const int *ns, cons
     if ((e = hipMemcpy(c->clu, hc.data(), sizeof(RClu) * nclu, hipMemcpyHostToDevice)) != hipSuccess) return bail("upload", e);
     if ((e = hipMemcpy(c->glob, &g, sizeof(RGlob), hipMemcpyHostToDevice)) != hipSuccess) return bail("upload", e);
     if ((e = hipMemset(c->jpr, 0, sizeof(double) * (size_t)Q * tot)) != hipSuccess) return bail("memset", e);
+    // four workgroups per CU, twice the two that are resident at once: a
+    // multiple of both the XCD count and the residency, and measured faster
+    // than a grid of exactly the resident slots
     c->force_grid = prop.multiProcessorCount * 4;
+    c->reduce_grid = prop.multiProcessorCount * 4;
     *out = c;
     return HH_OK;
 }

@@ -36,7 +36,7 @@ namespace hh {
 constexpr int R_DT = 26, R_ID = 27, R_D6A = 28, R_D7A = 31;
 __host__ __device__ constexpr int r_words(int o) { return o == 8 ? 34 : 28; }
 
-constexpr int R_GI = 4;          // i per group of the j-parallel task
+constexpr int R_GI = 2;          // i per group of the j-parallel task (half the wave sums of 4: short steps 30 % faster)
 // i per lane of the i-parallel task: two i share every LDS read of a j
 // (halving LDS traffic per interaction); the EXACT variant keeps one
 __host__ __device__ constexpr int r_ni(int mode) { return mode == EXACT ? 1 : 2; }
@@ -516,11 +516,15 @@ __device__ int r_find_cluster(const RClu *clu, int nclu, int task)
 }
 
 // Persistent force kernel: a fixed grid of workgroups walks the step's task
-// table, which holds tasks of either kind.  (Measured: one kernel with both
-// task kinds runs the tile tasks faster than a tile-only kernel with fewer
-// registers and twice the waves per SIMD.)
+// table, which holds tasks of either kind.  Measured choices: one kernel
+// with both task kinds runs the tile tasks faster than a tile-only kernel
+// with fewer registers and four waves per SIMD; a budget of 256 registers
+// (two waves per SIMD) is faster than 168 (three waves), which the 6th-order
+// tile loop fits only by giving up scheduling freedom and the 8th-order one
+// only by spilling; the 8th-order kernel, left alone, would take 288
+// registers (one wave per SIMD).
 template <int ORDER, int MODE>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void r_force_kernel(RArgs a)
+__device__ __forceinline__ void r_force_body(const RArgs &a)
 {
     constexpr int Q = pr_words(ORDER);
     constexpr int L = lds_words(ORDER);
@@ -721,6 +725,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void r_
             }
         }
     }
+}
+
+template <int ORDER, int MODE>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void r_force_kernel(RArgs a)
+{
+    r_force_body<ORDER, MODE>(a);
 }
 
 // Cluster of an active-list position: the last cluster whose act0 <= pos

@@ -175,3 +175,66 @@ def test_resident_force_timing_changes_nothing(use_graph):
     assert 0 < ms < 1e4
     a.close()
     b.close()
+
+
+def test_resident_exact_salpeter_bitwise_vs_oracle_inplace():
+    """Unequal (Salpeter) masses, 6th order: the resident EXACT schedule is
+    bit-identical to the oracle's in-place schedule."""
+    n, eps2, eta, eta_s = 512, (1 / 256) ** 2, 0.3, 0.1
+    m, x, v = H.plummer(n, 51)
+    m = H.salpeter(n, 5151)
+    x = x - (m[:, None] * x).sum(axis=0)
+    v = v - (m[:, None] * v).sum(axis=0)
+    s = O.System(6, O.make_particles(6, m, x, v), 0.0, eps2, eta, eta_s, 1 / 16)
+    s.init_step()
+    r = H.Resident(6, "exact", [(s.ptcl(), eps2, eta, 1 / 16, 0.0, 0.0)])
+    print("resident exact salpeter", flush=True)
+    s.integrate_until_inplace(1 / 16)
+    r.run(1 / 16, use_graph=True, batch=8)
+    st = r.stats()
+    assert (st["num_step"], st["num_bstep"]) == (s.num_step, s.num_bstep)
+    assert np.array_equal(r.particles(), s.ptcl())
+    r.close()
+
+
+def test_resident_dtmin_clamp():
+    """A lower step clamp: steps the quantised Aarseth criterion would make
+    shorter are clamped to dt_min and counted; no particle steps below it."""
+    order, n = 6, 512
+    eps2, eta, eta_s = (1 / 256) ** 2, 0.3, 0.1
+    o = initialised(order, n, 53, eps2, eta, eta_s, 1 / 16)
+    P0 = o.ptcl()
+    dtmin = float(np.median(P0[:, 3]))  # half the particles start below it
+    r = H.Resident(order, "fp64", [(P0, eps2, eta, 1 / 16, dtmin, 0.0)])
+    r.run(1 / 16)
+    st = r.stats()
+    P = r.particles()
+    assert st["dtmin_hits"] > 0
+    assert P[:, 3].min() >= dtmin
+    assert st["tsys"] == 1 / 16
+    r.close()
+
+
+@pytest.mark.parametrize("n", [3000, 9000])
+def test_resident_fp64_reduce_paths(n):
+    """Both slice-sum widths (a wave per particle while a step's active
+    particles fit one wave each, 8 lanes beyond) against the EXACT run of
+    the same cluster: the same schedule in the first steps and energies that
+    agree to rounding."""
+    order, eps2, eta, eta_s = 4, (4.0 / n) ** 2, 0.1, 0.01
+    o = initialised(order, n, 57, eps2, eta, eta_s, 1 / 16)
+    P0, e0 = o.ptcl(), o.init_energy
+    res = {}
+    for variant in ("exact", "fp64"):
+        r = H.Resident(order, variant, [(P0, eps2, eta, 1 / 16, 0.0, 0.0)])
+        r.set_timing(True)
+        r.run(1 / 32)
+        na, _, _ = r.step_log()
+        res[variant] = (r.stats(), r.energy(), na)
+        r.close()
+    (se, ee, nae), (sf, ef, naf) = res["exact"], res["fp64"]
+    print(f"n={n}: largest step {nae.max()} active, steps {se['num_bstep']} / {sf['num_bstep']}", flush=True)
+    if n > 4096:
+        assert nae.max() > 4096  # the 8-lane path ran
+    assert abs(sf["num_step"] / se["num_step"] - 1) < 0.01
+    assert abs((ef - ee) / e0) < 1e-9
