@@ -87,16 +87,19 @@ def initial_states(w):
     return out
 
 
-def integrate(w, variant, states, tend, timing=False, graph=False, chunks=1, wall_limit=None):
+def integrate(w, variant, states, tend, timing=False, graph=False, chunks=1, wall_limit=None, max_bsteps=0):
     """Resident integration t = 0 -> tend, in `chunks` equal time spans
     (each hhr_run call ends with a stream synchronisation; a progress line
     per chunk).  With wall_limit (s) the run stops after the first chunk
-    that ends past it."""
+    that ends past it; with max_bsteps a chunk stops after that many block
+    steps."""
     clus = [(P0, eps2, w.eta, w.dtmax, w.dtmin, 0.0) for P0, eps2, _, _ in states]
     r = H.Resident(w.order, variant, clus)
     try:
         if timing:
             r.set_timing(True)
+        if max_bsteps:
+            r.set_step_limit(max_bsteps)
         t0 = time.perf_counter()
         err = None
         try:
@@ -193,8 +196,8 @@ def cpu_baseline(w, states, cpu_tend, budget_s=20.0):
     """The reference's own driver and AVX2 back end (SRC/nbodysystem.h with
     SRC/hermite{4,6,8}-avx.h, built from its sources with its own flags into
     oracle/_ref) on the same initial states.  Single systems of up to 2^16
-    particles integrate t = 0 -> cpu_tend; larger ones time all-pairs force
-    calls on 4096 active particles; the ensemble integrates its clusters in
+    particles integrate t = 0 -> cpu_tend; larger ones time about 10 s of
+    force calls on 4096 active particles; the ensemble integrates its clusters in
     order until about budget_s seconds of CPU work."""
     try:
         import oracle
@@ -214,11 +217,13 @@ def cpu_baseline(w, states, cpu_tend, budget_s=20.0):
     elif len(states) == 1:
         P0, eps2 = states[0][0], states[0][1]
         nact = 4096
-        _, info = ref.forces(P0, 0.0, nact, eps2, reps=2, threads=threads, want_output=False)
-        inter = 2.0 * nact * len(P0)
+        _, info = ref.forces(P0, 0.0, nact, eps2, reps=1, threads=threads, want_output=False)
+        reps = int(min(200, max(2, 10.0 / max(info["seconds"], 1e-3))))  # about 10 s of calls
+        _, info = ref.forces(P0, 0.0, nact, eps2, reps=reps, threads=threads, want_output=False)
+        inter = float(reps) * nact * len(P0)
         secs, used = info["seconds"], info["threads"]
         sample = (f"reference AVX2 back end, force on {nact} active particles from all N={len(P0)} of the same "
-                  f"state, 2 calls ({secs:.1f} s), {cpu}")
+                  f"state, {reps} calls ({secs:.1f} s), {cpu}")
     else:
         inter, secs, used, done = 0.0, 0.0, threads, 0
         for P0, eps2, _, P_raw in states:
@@ -329,11 +334,12 @@ def main():
     if not args.no_split:
         # 6th and 8th order do not integrate with the FP32 body (step
         # collapse, or steps pinned at dt_min when there is a clamp): one
-        # interval at most, in small chunks, for at most 30 s
+        # interval at most, in small chunks, for at most 30 s and 1024
+        # block steps
         split_tend = tend if w.order == 4 else w.dtmax
         log(f"split-precision integration t=0->{split_tend:g}")
         s = integrate(w, "split", states, split_tend, chunks=chunks if w.order == 4 else 256,
-                      wall_limit=None if w.order == 4 else 30.0)
+                      wall_limit=None if w.order == 4 else 30.0, max_bsteps=0 if w.order == 4 else 1024)
         log(f"split: t={s['tsys']:.3e} after {s['num_bstep']} block steps: {s['error'] or 'ok'}")
         split = {"integration_error": s["error"], "tsys_reached": s["tsys"], "block_steps": s["num_bstep"],
                  "interactions_per_s": s["interactions"] / s["wall_s"], "dE_E": s.get("dE_E"),

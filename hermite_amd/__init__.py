@@ -35,6 +35,11 @@ class HHError(RuntimeError):
     pass
 
 
+class StepLimitReached(HHError):
+    """Resident.run stopped at its block-step limit (HH_E_LIMIT); the state
+    is that of a completed block step and a later run continues."""
+
+
 _hip = None
 _host = None
 
@@ -85,6 +90,7 @@ def hip_lib():
         L.hhr_set_timing.argtypes = [vp, C.c_int]
         L.hhr_set_plan.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
         L.hhr_set_split_waves.argtypes = [vp, C.c_int, C.c_int]
+        L.hhr_set_step_limit.argtypes = [vp, C.c_longlong]
         L.hhr_force_timing.argtypes = [vp, _dp, C.POINTER(C.c_longlong)]
         L.hhr_step_log.argtypes = [vp, C.c_longlong, C.POINTER(C.c_int), C.POINTER(C.c_float),
                                    C.POINTER(C.c_float)]
@@ -456,8 +462,16 @@ class Resident:
         self.close()
 
     def run(self, tend, use_graph=True, batch=8):
-        self._check(hip_lib().hhr_run(self.h, tend, int(use_graph), batch), "hhr_run")
+        rc = hip_lib().hhr_run(self.h, tend, int(use_graph), batch)
+        if rc == -5:
+            raise StepLimitReached(f"hhr_run: {hip_lib().hhr_last_error(self.h).decode()}")
+        self._check(rc, "hhr_run")
         return hip_lib().hhr_last_run_ms(self.h)
+
+    def set_step_limit(self, max_block_steps=0):
+        """Make run() stop (StepLimitReached) once it has launched this many
+        block steps without reaching tend; 0 removes the limit."""
+        self._check(hip_lib().hhr_set_step_limit(self.h, int(max_block_steps)), "hhr_set_step_limit")
 
     def stats(self, k=0):
         a, b, d = C.c_longlong(), C.c_longlong(), C.c_longlong()

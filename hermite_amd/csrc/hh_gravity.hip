@@ -663,6 +663,7 @@ struct hhr_ctx {
     int graph_steps = 0;
     double last_run_ms = 0.0;
     long long steps_launched = 0;
+    long long step_limit = 0;  // block steps per hhr_run call, 0 = none
     // Optional per-step timing: four events per step of a batch (step start,
     // before and after the force launch, step end).  Timed runs launch their
     // steps directly on the stream (no graph), so the events are ordinary
@@ -986,6 +987,7 @@ int hhr_run(hhr_ctx *c, double tend, int use_graph, int batch_steps)
         c->graph_steps = batch_steps;
     }
     RGlob g;
+    long long launched = 0;
     RCHK(c, hipEventRecord(c->ev0, c->stream));
     for (;;) {
         if (use_graph) {
@@ -997,6 +999,7 @@ int hhr_run(hhr_ctx *c, double tend, int use_graph, int batch_steps)
             }
         }
         c->steps_launched += batch_steps;
+        launched += batch_steps;
         RCHK(c, hipMemcpyAsync(&g, c->glob, sizeof(RGlob), hipMemcpyDeviceToHost, c->stream));
         RCHK(c, hipStreamSynchronize(c->stream));
         if (c->timing) {
@@ -1022,6 +1025,9 @@ int hhr_run(hhr_ctx *c, double tend, int use_graph, int batch_steps)
         if (g.error == 1) return rfail(c, HH_E_STATE, "a cluster's block time stopped advancing (step-size collapse)");
         if (g.error == 2) return rfail(c, HH_E_STATE, "a particle state became non-finite");
         if (g.nactive == 0) break;
+        if (c->step_limit > 0 && launched >= c->step_limit)
+            return rfail(c, HH_E_LIMIT, "stopped at the step limit (" + std::to_string(launched) +
+                                            " block steps launched in this call)");
     }
     RCHK(c, hipEventRecord(c->ev1, c->stream));
     RCHK(c, hipEventSynchronize(c->ev1));
@@ -1071,6 +1077,13 @@ int hhr_set_split_waves(hhr_ctx *c, int wj4_max, int wj2_max)
     const int v[2] = {wj4_max, wj2_max};
     RCHK(c, hipMemcpy((char *)c->glob + offsetof(RGlob, plan) + offsetof(RPlan, wj4_max), v, sizeof(v),
                       hipMemcpyHostToDevice));
+    return HH_OK;
+}
+
+int hhr_set_step_limit(hhr_ctx *c, long long max_block_steps)
+{
+    if (!c || max_block_steps < 0) return HH_E_ARG;
+    c->step_limit = max_block_steps;
     return HH_OK;
 }
 

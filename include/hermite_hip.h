@@ -68,7 +68,8 @@ enum {
     HH_E_ARG = -1,     /* invalid argument (order, size, null pointer) */
     HH_E_HIP = -2,     /* HIP runtime error */
     HH_E_NODEV = -3,   /* no usable gfx950 device */
-    HH_E_STATE = -4    /* call out of sequence */
+    HH_E_STATE = -4,   /* call out of sequence */
+    HH_E_LIMIT = -5    /* hhr_run stopped at its block-step limit */
 };
 
 /* Doubles per record at the boundary and on the device. */
@@ -192,8 +193,15 @@ int hhr_stats(hhr_ctx *ctx, int k, long long *num_step, long long *num_bstep, do
 /* Wall milliseconds of the last hhr_run and block steps it launched. */
 double hhr_last_run_ms(const hhr_ctx *ctx);
 long long hhr_steps_launched(const hhr_ctx *ctx);
+/* Block-step limit of hhr_run (no reference counterpart; 0, the default,
+ * is none): a call that has launched max_block_steps or more steps without
+ * reaching tend returns HH_E_LIMIT after the batch in flight.  The state is
+ * that of a completed block step, so a later hhr_run continues the
+ * integration; for bounded runs of integrations that may crawl (steps
+ * pinned at dt_min). */
+int hhr_set_step_limit(hhr_ctx *ctx, long long max_block_steps);
 /* Work split of the persistent force kernel (no reference counterpart):
- * active counts up to jpar_max use j-parallel groups of 4 i; larger ones
+ * active counts up to jpar_max use j-parallel groups of 2 i; larger ones
  * i-parallel groups of 256 with at most smax j-slices; a block step aims at
  * task_target i-parallel (jpar_target j-parallel) tasks over all clusters
  * (both at most 8192).  Defaults 64, 256, 4096, 256 (measured best for a

@@ -238,3 +238,32 @@ def test_resident_fp64_reduce_paths(n):
         assert nae.max() > 4096  # the 8-lane path ran
     assert abs(sf["num_step"] / se["num_step"] - 1) < 0.01
     assert abs((ef - ee) / e0) < 1e-9
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_resident_step_limit_stops_and_resumes(use_graph):
+    """A block-step limit stops run() after the batch that reaches it, at a
+    completed block step; removing the limit and running on gives the state
+    of an uninterrupted run, bit for bit."""
+    order, n, eps2 = 6, 500, (1 / 32) ** 2
+    o = initialised(order, n, 61, eps2, 0.4, 0.1, 1 / 16)
+    P0 = o.ptcl()
+    print(f"resident step limit graph={use_graph}", flush=True)
+    a = H.Resident(order, "fp64", [(P0, eps2, 0.4, 1 / 16, 0.0, 0.0)])
+    a.run(1 / 8, use_graph=use_graph, batch=8)
+    full = a.stats()["num_bstep"]
+    assert full > 40
+    b = H.Resident(order, "fp64", [(P0, eps2, 0.4, 1 / 16, 0.0, 0.0)])
+    b.set_step_limit(20)
+    with pytest.raises(H.StepLimitReached):
+        b.run(1 / 8, use_graph=use_graph, batch=8)
+    st = b.stats()
+    assert 20 <= st["num_bstep"] <= 24 and st["tsys"] < 1 / 8
+    b.set_step_limit(0)
+    b.run(1 / 8, use_graph=use_graph, batch=8)
+    assert a.stats() == b.stats()
+    assert np.array_equal(a.particles(), b.particles())
+    with pytest.raises(H.HHError):
+        b.set_step_limit(-1)
+    a.close()
+    b.close()
