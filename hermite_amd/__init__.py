@@ -74,6 +74,7 @@ def hip_lib():
         L.hh_predict_async.argtypes = [vp, C.c_double]
         L.hh_synchronize.argtypes = [vp]
         L.hh_last_geometry.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.hh_eval_rsqrt.argtypes = [vp, C.c_int, _dp, _dp]
         L.hhr_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), _dp, _dp, C.POINTER(vp)]
         L.hhr_destroy.argtypes = [vp]
         L.hhr_run.argtypes = [vp, C.c_double, C.c_int, C.c_int]
@@ -338,6 +339,13 @@ class Gravity:
         k, s, w = C.c_int(), C.c_int(), C.c_int()
         hip_lib().hh_last_geometry(self.h, C.byref(k), C.byref(s), C.byref(w))
         return {"kernel": k.value, "jsplit": s.value, "workgroups": w.value}
+
+    def eval_rsqrt(self, x):
+        """1/sqrt(x) as the FP64 kernels compute it, on this context's device."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        self._check(hip_lib().hh_eval_rsqrt(self.h, len(x), _p(x), _p(y)), "hh_eval_rsqrt")
+        return y
 
 
 # ------------------------------------------------------------------ driver

@@ -60,6 +60,14 @@ static int fail(hh_ctx *c, int code, const std::string &msg)
     return code;
 }
 
+// The FP64 kernels' r^-1 (rsqrt64) element-wise, for checking it against a
+// correctly rounded reference.
+__global__ void rsqrt_eval_kernel(const double *__restrict__ x, double *__restrict__ y, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = hh::rsqrt64(x[i]);
+}
+
 #define HIPCHK(c, call)                                                                     \
     do {                                                                                    \
         hipError_t e_ = (call);                                                             \
@@ -624,6 +632,28 @@ int hh_synchronize(hh_ctx *c)
     int r = check_ctx(c);
     if (r) return r;
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return HH_OK;
+}
+
+int hh_eval_rsqrt(hh_ctx *c, int n, const double *x, double *y)
+{
+    int r = check_ctx(c);
+    if (r) return r;
+    if (n < 0 || (n > 0 && (!x || !y))) return fail(c, HH_E_ARG, "bad rsqrt arguments");
+    if (n == 0) return HH_OK;
+    double *d = nullptr;
+    HIPCHK(c, hipMalloc(&d, sizeof(double) * 2 * (size_t)n));
+    const size_t bytes = sizeof(double) * (size_t)n;
+    hipError_t e = hipMemcpyAsync(d, x, bytes, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        rsqrt_eval_kernel<<<(n + NT - 1) / NT, NT, 0, c->stream>>>(d, d + n, n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(y, d + n, bytes, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    const hipError_t ef = hipFree(d);
+    if (e == hipSuccess) e = ef;
+    if (e != hipSuccess) return fail(c, HH_E_HIP, std::string("hh_eval_rsqrt: ") + hipGetErrorString(e));
     return HH_OK;
 }
 

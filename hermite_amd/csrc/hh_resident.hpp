@@ -1,6 +1,6 @@
 // GPU-resident block-timestep integration of one or many independent
 // clusters (ensembles): the whole block step of SRC/nbodysystem.h:692-765
-// runs on the device, so a step costs five kernel launches (capturable in a
+// runs on the device, so a step costs four kernel launches (capturable in a
 // hipGraph) and no host round trip.
 //
 // Per block step:

@@ -158,6 +158,14 @@ int hh_force_async(hh_ctx *ctx, int nact, double eps2, int with_pot);
 int hh_predict_async(hh_ctx *ctx, double tsys);
 int hh_synchronize(hh_ctx *ctx);
 
+/* y[i] = 1/sqrt(x[i]) as the FP64 kernels compute it (v_rsq_f32 seed and
+ * a second-order refinement; valid for FLT_MIN < x < FLT_MAX), on the
+ * context's device (no reference counterpart: the reference's rsqrtf-seeded
+ * refinement, SRC/cuda-common.hu:72-79, has no entry point; this one lets
+ * tests bound the error against a correctly rounded 1/sqrt).  Host arrays of
+ * n doubles; blocking. */
+int hh_eval_rsqrt(hh_ctx *ctx, int n, const double *x, double *y);
+
 /* Geometry the last force call used: kernel id, j-slices, workgroups. */
 int hh_last_geometry(const hh_ctx *ctx, int *kernel, int *jsplit, int *workgroups);
 

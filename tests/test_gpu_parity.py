@@ -269,3 +269,27 @@ def test_tuning_table_selects_geometry(variant, tmp_path):
     g.calc_force_on_first_nact(200, eps2)
     assert g.last_geometry()["jsplit"] == 8
     g.close()
+
+
+def test_rsqrt64_error_bound():
+    """The FP64 kernels' r^-1 (v_rsq_f32 seed, second-order series
+    refinement) against 1/sqrt in x87 extended precision (64-bit
+    significand) over a log-uniform sweep 2^-120 .. 2^120: at most about
+    half an ulp (the refinement's truncation is ~2^-70), a quarter ulp on
+    average, and exact where 1/sqrt(x) is a power of two."""
+    rng = np.random.default_rng(2024)
+    x = np.exp2(rng.uniform(-120.0, 120.0, 1 << 20))
+    exact_x = np.exp2(np.arange(-120, 121, 2, dtype=np.float64))
+    g = H.Gravity(6, 64, "fp64")
+    try:
+        y = g.eval_rsqrt(x)
+        ye = g.eval_rsqrt(exact_x)
+    finally:
+        g.close()
+    ref = 1.0 / np.sqrt(x.astype(np.longdouble))
+    ulp = np.spacing(ref.astype(np.float64))
+    err = (np.abs(y.astype(np.longdouble) - ref) / ulp).astype(np.float64)
+    print(f"rsqrt64: max {err.max():.4f} ulp, mean {err.mean():.4f} ulp", flush=True)
+    assert err.max() <= 0.51
+    assert err.mean() <= 0.3
+    assert np.array_equal(ye, np.exp2(-np.arange(-120, 121, 2) / 2.0))
