@@ -293,3 +293,32 @@ def test_rsqrt64_error_bound():
     assert err.max() <= 0.51
     assert err.mean() <= 0.3
     assert np.array_equal(ye, np.exp2(-np.arange(-120, 121, 2) / 2.0))
+
+
+@pytest.mark.parametrize("order", ORDERS)
+@pytest.mark.parametrize("n", [1, 2, 3, 63, 64, 65, 255, 256, 257, 513])
+def test_edge_sizes(order, n):
+    """Tiny and ragged systems: N below, at and just above a 64-lane wave
+    and a 256-particle tile.  EXACT forces on the first particle and on all
+    N, and the potential, bit-identical to the oracle; FP64 forces within
+    the FP64 tolerance.  N = 1 is a lone particle: its self term is exactly
+    zero."""
+    eps2 = (1 / 64) ** 2
+    P = hermite_state(order, n, seed=17)
+    R = O.force(order, O.predict(order, P, 0.0), n, eps2)
+    print(f"edge order={order} n={n}", flush=True)
+    for nact in sorted({1, n}):
+        F = gpu_forces(order, P, "exact", 0.0, nact, eps2)
+        assert np.array_equal(F, R[:nact])
+    g = H.Gravity(order, n, "exact")
+    try:
+        g.set_jp_all(P)
+        assert np.array_equal(g.calc_potential(eps2), O.potential(order, P, eps2))
+    finally:
+        g.close()
+    F = gpu_forces(order, P, "fp64", 0.0, n, eps2)
+    if n == 1:
+        assert not F.any()
+        return
+    errs = fam_err(F, R, order)
+    assert errs[0].max() <= 1e-12 and errs[1].max() <= 1e-12

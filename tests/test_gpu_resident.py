@@ -267,3 +267,33 @@ def test_resident_step_limit_stops_and_resumes(use_graph):
         b.set_step_limit(-1)
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("n", [2, 3, 65, 257])
+def test_resident_edge_sizes(n):
+    """Clusters of 2, 3, 65 and 257 particles (less than a wave, just over
+    a wave and a tile): the EXACT resident schedule is bit-identical to the
+    oracle's in-place schedule, and an ensemble of all four sizes gives each
+    cluster what it gives alone (FP64)."""
+    eps2, eta, eta_s = (1 / 32) ** 2, 0.4, 0.1
+    o = initialised(6, n, 71, eps2, eta, eta_s, 1 / 16)
+    r = H.Resident(6, "exact", [(o.ptcl(), eps2, eta, 1 / 16, 0.0, 0.0)])
+    print(f"resident edge n={n}", flush=True)
+    o.integrate_until_inplace(1 / 8)
+    r.run(1 / 8, use_graph=True, batch=4)
+    st = r.stats()
+    assert (st["num_step"], st["num_bstep"], st["tsys"]) == (o.num_step, o.num_bstep, o.tsys)
+    assert np.array_equal(r.particles(), o.ptcl())
+    r.close()
+    if n != 257:
+        return
+    clus = [initialised(6, m, 71, eps2, eta, eta_s, 1 / 16).ptcl() for m in (2, 3, 65, 257)]
+    ens = H.Resident(6, "fp64", [(P, eps2, eta, 1 / 16, 0.0, 0.0) for P in clus])
+    ens.run(1 / 8, use_graph=True, batch=4)
+    for k, P in enumerate(clus):
+        one = H.Resident(6, "fp64", [(P, eps2, eta, 1 / 16, 0.0, 0.0)])
+        one.run(1 / 8, use_graph=True, batch=4)
+        assert one.stats() == ens.stats(k)
+        assert np.array_equal(one.particles(), ens.particles(k))
+        one.close()
+    ens.close()
