@@ -505,7 +505,7 @@ class Resident:
         """Work split of the persistent force kernel (see hhr_set_plan)."""
         self._check(hip_lib().hhr_set_plan(self.h, jpar_max, smax, task_target, jpar_target), "hhr_set_plan")
 
-    def set_split_waves(self, wj4_max=1024, wj2_max=2048):
+    def set_split_waves(self, wj4_max=8192, wj2_max=16384):
         """Active-count thresholds below which i-parallel tasks split each
         j-tile over 4 (2) waves (see hhr_set_split_waves)."""
         self._check(hip_lib().hhr_set_split_waves(self.h, wj4_max, wj2_max), "hhr_set_split_waves")

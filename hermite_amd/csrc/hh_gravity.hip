@@ -1095,7 +1095,8 @@ int hhr_set_plan(hhr_ctx *c, int jpar_max, int smax, int task_target, int jpar_t
     p.smax = smax;
     p.task_target = task_target;
     p.jpar_target = jpar_target;
-    RCHK(c, hipMemcpy((char *)c->glob + offsetof(RGlob, plan), &p, sizeof(RPlan), hipMemcpyHostToDevice));
+    // the split-wave thresholds that follow are hhr_set_split_waves' to set
+    RCHK(c, hipMemcpy((char *)c->glob + offsetof(RGlob, plan), &p, offsetof(RPlan, wj4_max), hipMemcpyHostToDevice));
     return HH_OK;
 }
 

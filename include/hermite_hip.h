@@ -218,7 +218,7 @@ int hhr_set_plan(hhr_ctx *ctx, int jpar_max, int smax, int task_target, int jpar
 /* i-parallel tasks of clusters with at most wj4_max (wj2_max) active
  * particles split each j-tile over 4 (2) waves of the workgroup, so a task
  * covers 128 (256) instead of 512 i (no reference counterpart; defaults
- * 1024 and 2048; EXACT always uses whole tiles per wave). */
+ * 8192 and 16384; EXACT always uses whole tiles per wave). */
 int hhr_set_split_waves(hhr_ctx *ctx, int wj4_max, int wj2_max);
 
 /* Force-kernel timing (no reference counterpart): when on, hhr_run launches

@@ -297,3 +297,28 @@ def test_resident_edge_sizes(n):
         assert np.array_equal(one.particles(), ens.particles(k))
         one.close()
     ens.close()
+
+
+def test_plan_and_split_wave_settings_are_independent():
+    """hhr_set_plan leaves the split-wave thresholds alone: setting them
+    before or after the plan gives the same integration, bit for bit, and
+    whole-tile tasks (thresholds 0) sum j in another order than the default
+    split tiles."""
+    order, n, eps2 = 6, 3000, (1 / 64) ** 2
+    o = initialised(order, n, 67, eps2, 0.2, 0.1, 1 / 16)
+    P0 = o.ptcl()
+    print("plan / split-wave settings", flush=True)
+    out = []
+    for how in ("split-then-plan", "plan-then-split", "default"):
+        r = H.Resident(order, "fp64", [(P0, eps2, 0.2, 1 / 16, 0.0, 0.0)])
+        if how == "split-then-plan":
+            r.set_split_waves(0, 0)
+            r.set_plan()
+        elif how == "plan-then-split":
+            r.set_plan()
+            r.set_split_waves(0, 0)
+        r.run(1 / 32, use_graph=True, batch=8)
+        out.append((r.stats(), r.particles()))
+        r.close()
+    assert out[0][0] == out[1][0] and np.array_equal(out[0][1], out[1][1])
+    assert not np.array_equal(out[0][1], out[2][1])
