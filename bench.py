@@ -33,6 +33,7 @@ Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
 import json
+import math
 import os
 import platform
 import subprocess
@@ -192,6 +193,11 @@ def _cpu_model():
     return cpu
 
 
+def _reps_for(secs, target_s=10.0, most=2000):
+    """Repetitions that bring a CPU sample of secs seconds to about target_s."""
+    return int(min(most, max(1, math.ceil(target_s / max(secs, 1e-3))))) if secs < 0.5 * target_s else 1
+
+
 def cpu_baseline(w, states, cpu_tend, budget_s=20.0):
     """The reference's own driver and AVX2 back end (SRC/nbodysystem.h with
     SRC/hermite{4,6,8}-avx.h, built from its sources with its own flags into
@@ -209,10 +215,13 @@ def cpu_baseline(w, states, cpu_tend, budget_s=20.0):
     if len(states) == 1 and len(states[0][3]) <= 65536:
         P_raw, eps2 = states[0][3], states[0][1]
         info = ref.run(P_raw, eps2, w.eta, w.eta_s, w.dtmax, cpu_tend, threads=threads)
-        inter = float(info["num_step"]) * len(P_raw)
+        reps = _reps_for(info["seconds"])
+        if reps > 1:  # a short run is repeated from the same state
+            info = ref.run(P_raw, eps2, w.eta, w.eta_s, w.dtmax, cpu_tend, threads=threads, reps=reps)
+        inter = float(reps) * info["num_step"] * len(P_raw)
         sample = (f"reference driver + AVX2 back end, same N={len(P_raw)} state, t=0->{cpu_tend:g} "
-                  f"({info['num_bstep']} block steps, {info['num_step']} particle steps, {info['seconds']:.1f} s, "
-                  f"dE/E {info['de']:.2e}), {cpu}")
+                  f"({info['num_bstep']} block steps, {info['num_step']} particle steps, dE/E {info['de']:.2e})"
+                  + (f" run {reps} times" if reps > 1 else "") + f" ({info['seconds']:.1f} s), {cpu}")
         secs, used = info["seconds"], info["threads"]
     elif len(states) == 1:
         P0, eps2 = states[0][0], states[0][1]
@@ -225,20 +234,29 @@ def cpu_baseline(w, states, cpu_tend, budget_s=20.0):
         sample = (f"reference AVX2 back end, force on {nact} active particles from all N={len(P0)} of the same "
                   f"state, {reps} calls ({secs:.1f} s), {cpu}")
     else:
-        inter, secs, used, done = 0.0, 0.0, threads, 0
-        for P0, eps2, _, P_raw in states:
-            if len(P_raw) % 4:  # the AVX back end takes i in groups of 4 (SRC/hermite4-avx.h:289)
-                continue
-            info = ref.run(P_raw, eps2, w.eta, w.eta_s, w.dtmax, cpu_tend, threads=threads)
-            inter += float(info["num_step"]) * len(P_raw)
-            secs += info["seconds"]
-            used = info["threads"]
-            done += 1
-            if secs > budget_s:
-                break
+        def ensemble_pass(reps, limit):
+            inter, secs, used, done = 0.0, 0.0, threads, 0
+            for P0, eps2, _, P_raw in states:
+                if len(P_raw) % 4:  # the AVX back end takes i in groups of 4 (SRC/hermite4-avx.h:289)
+                    continue
+                if limit is not None and done >= limit:
+                    break
+                info = ref.run(P_raw, eps2, w.eta, w.eta_s, w.dtmax, cpu_tend, threads=threads, reps=reps)
+                inter += float(reps) * info["num_step"] * len(P_raw)
+                secs += info["seconds"]
+                used = info["threads"]
+                done += 1
+                if secs > budget_s:
+                    break
+            return inter, secs, used, done
+
+        inter, secs, used, done = ensemble_pass(1, None)
+        reps = _reps_for(secs)
+        if reps > 1:  # the same clusters again, each run reps times
+            inter, secs, used, done = ensemble_pass(reps, done)
         sample = (f"reference driver + AVX2 back end on the first {done} clusters of the ensemble whose N_k is "
-                  f"a multiple of 4, each "
-                  f"t=0->{cpu_tend:g} ({secs:.1f} s), {cpu}")
+                  f"a multiple of 4, each t=0->{cpu_tend:g}" + (f" run {reps} times" if reps > 1 else "")
+                  + f" ({secs:.1f} s), {cpu}")
     return {"value": inter / secs, "unit": "interactions/s", "cores": used, "kind": "reference",
             "sample": sample, "gflops": inter / secs * H.FLOPS[w.order] * 1e-9}
 

@@ -285,10 +285,12 @@ class RefAVX:
             self._run(["pot", fin, len(P), repr(float(eps2)), fout])
             return np.fromfile(fout)
 
-    def run(self, P, eps2, eta, eta_s, dtmax, tend, threads=None):
+    def run(self, P, eps2, eta, eta_s, dtmax, tend, threads=None, reps=1):
         """The reference driver on the AVX2 back end from t = 0 (init_step,
-        then integrate_one_dtmax until tend; only the integration is timed).
-        Returns {e0, e1, de, num_step, num_bstep, seconds, threads}."""
+        then integrate_one_dtmax until tend; only the integration is timed),
+        reps times from the same state.  Returns {e0, e1, de, num_step,
+        num_bstep, seconds, reps, threads}: seconds summed over the reps,
+        the rest of one run."""
         import tempfile
 
         env = dict(os.environ)
@@ -298,7 +300,8 @@ class RefAVX:
         with tempfile.TemporaryDirectory() as td:
             fin = os.path.join(td, "p.bin")
             P.tofile(fin)
-            return self._run(["run", fin, len(P)] + [repr(float(a)) for a in (eps2, eta, eta_s, dtmax, tend)], env)
+            return self._run(["run", fin, len(P)] + [repr(float(a)) for a in (eps2, eta, eta_s, dtmax, tend)]
+                             + [int(reps)], env)
 
 
 class Ref:

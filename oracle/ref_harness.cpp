@@ -150,6 +150,7 @@ void ref_sys_get_pot(void *s, double *pot)
 #include <cstdlib>
 #include <string>
 #include <vector>
+#include <algorithm>
 #include <omp.h>
 
 static std::vector<double> read_recs(const char *path, int n)
@@ -219,22 +220,30 @@ int main(int argc, char **argv)
         ref_gravity_free(g);
         return 0;
     }
-    if (cmd == "run" && argc == 9) {
+    if (cmd == "run" && (argc == 9 || argc == 10)) {
+        // the same integration from the same initial state, reps times (only
+        // the integrations are timed; steps and energies are one run's)
         const int n = std::atoi(argv[3]);
+        const int reps = argc == 10 ? std::max(1, std::atoi(argv[9])) : 1;
         std::vector<double> recs = read_recs(argv[2], n);
-        void *s = ref_sys_new(n, recs.data(), 0.0, std::atof(argv[4]), std::atof(argv[5]),
-                              std::atof(argv[6]), std::atof(argv[7]), std::atof(argv[8]));
-        ref_sys_init_step(s);
-        const double e0 = ref_sys_init_energy(s);
-        const double t0 = omp_get_wtime();
-        while (ref_sys_tsys(s) < std::atof(argv[8])) ref_sys_integrate_one_dtmax(s);
-        const double t1 = omp_get_wtime();
-        const double e1 = ref_sys_energy(s);
+        double secs = 0.0, e0 = 0.0, e1 = 0.0;
+        long nstep = 0, nbstep = 0;
+        for (int r = 0; r < reps; r++) {
+            void *s = ref_sys_new(n, recs.data(), 0.0, std::atof(argv[4]), std::atof(argv[5]),
+                                  std::atof(argv[6]), std::atof(argv[7]), std::atof(argv[8]));
+            ref_sys_init_step(s);
+            e0 = ref_sys_init_energy(s);
+            const double t0 = omp_get_wtime();
+            while (ref_sys_tsys(s) < std::atof(argv[8])) ref_sys_integrate_one_dtmax(s);
+            secs += omp_get_wtime() - t0;
+            e1 = ref_sys_energy(s);
+            nstep = ref_sys_num_step(s);
+            nbstep = ref_sys_num_bstep(s);
+            ref_sys_free(s);
+        }
         std::printf("{\"e0\": %.17e, \"e1\": %.17e, \"de\": %.6e, \"num_step\": %ld, \"num_bstep\": %ld, "
-                    "\"seconds\": %.6e, \"threads\": %d}\n",
-                    e0, e1, (e0 - e1) / e0, ref_sys_num_step(s), ref_sys_num_bstep(s), t1 - t0,
-                    omp_get_max_threads());
-        ref_sys_free(s);
+                    "\"seconds\": %.6e, \"reps\": %d, \"threads\": %d}\n",
+                    e0, e1, (e0 - e1) / e0, nstep, nbstep, secs, reps, omp_get_max_threads());
         return 0;
     }
     std::fprintf(stderr, "bad command\n");
