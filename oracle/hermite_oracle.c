@@ -1,7 +1,7 @@
 /*
  * CPU FP64 oracle for the Hermite force + predictor path.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker: only tests/, the
+ * TEST INFRASTRUCTURE ONLY.  This file is the oracle: only tests/, the
  * smoke check in __graft_entry__.py and the cpu_baseline leg of bench.py may
  * load it.  The product (hermite_amd/) never links or calls it.
  *

@@ -5,7 +5,7 @@ to the reference CPU build's) and reproduces the nact known answers printed
 in the reference's run logs.  FP64 / SPLIT: last-bit force differences move
 individual timestep levels, so the comparison is statistical: mean nact
 within 1 % of the known answer and |dE/E| of the same size as the oracle's
-(the split kernel within 2x of the FP64 kernel)."""
+(the split kernel within 2x of the FP64 kernel, the stated bound)."""
 import numpy as np
 import pytest
 

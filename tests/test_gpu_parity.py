@@ -1,12 +1,12 @@
 """Parity of the HIP back end (called through the C-ABI) with the CPU oracle.
 
-Tolerances (stated per variant, from the north-star requirements):
+Tolerances (stated per variant):
   EXACT  bit-identical to the oracle (which is itself bit-identical to the
          reference's generic CPU back end, tests/test_oracle_ref.py).
   FP64   per particle, |F_gpu - F_oracle| / |F_oracle| per vector family:
          acc, jrk <= 1e-12; snp, crk <= 1e-10 (these two suffer cancellation,
          sum |terms| >> |sum|, so they are also checked against sum|terms|).
-  SPLIT  per-particle relative force (acc) error <= 1e-6 (north star);
+  SPLIT  per-particle relative force (acc) error <= 1e-6 (the target bound);
          jrk <= 5e-6 (FP32 rounding of the pair terms is amplified by the
          cancellation inside dv - 3 (dr.dv/r^2) dr); snp/crk median <= 1e-5.
 """
