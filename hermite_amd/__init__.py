@@ -501,7 +501,7 @@ class Resident:
     def steps_launched(self):
         return hip_lib().hhr_steps_launched(self.h)
 
-    def set_plan(self, jpar_max=64, smax=256, task_target=4096, jpar_target=256):
+    def set_plan(self, jpar_max=64, smax=256, task_target=8192, jpar_target=256):
         """Work split of the persistent force kernel (see hhr_set_plan)."""
         self._check(hip_lib().hhr_set_plan(self.h, jpar_max, smax, task_target, jpar_target), "hhr_set_plan")
 

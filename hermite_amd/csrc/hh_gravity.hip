@@ -960,10 +960,12 @@ int hhr_create(int order, int variant, int device, int nclu, const int *ns, cons
     if ((e = hipMemcpy(c->clu, hc.data(), sizeof(RClu) * nclu, hipMemcpyHostToDevice)) != hipSuccess) return bail("upload", e);
     if ((e = hipMemcpy(c->glob, &g, sizeof(RGlob), hipMemcpyHostToDevice)) != hipSuccess) return bail("upload", e);
     if ((e = hipMemset(c->jpr, 0, sizeof(double) * (size_t)Q * tot)) != hipSuccess) return bail("memset", e);
-    // four workgroups per CU, twice the two that are resident at once: a
-    // multiple of both the XCD count and the residency, and measured faster
-    // than a grid of exactly the resident slots
-    c->force_grid = prop.multiProcessorCount * 4;
+    // force: sixteen workgroups per CU, eight times the two resident at
+    // once, so that a step's tasks (up to the 8192 of the plan's target)
+    // mostly get a workgroup each and the dispatcher balances them as
+    // workgroups finish; measured faster than 4 per CU (C2 1 %, C4 4 %) and
+    // than 32.  reduce: four per CU (16 is slower)
+    c->force_grid = prop.multiProcessorCount * 16;
     c->reduce_grid = prop.multiProcessorCount * 4;
     *out = c;
     return HH_OK;

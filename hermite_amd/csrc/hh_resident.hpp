@@ -52,7 +52,7 @@ constexpr int R_SLOTS = 32;      // words a block-time minimum is spread over
 // workgroup's waves, so an i-parallel group is 64 (128) i per lane-slot
 // row instead of 256: short lists waste fewer lanes.
 struct RPlan {
-    int jpar_max = 64, smax = 256, task_target = 4096, jpar_target = 256;
+    int jpar_max = 64, smax = 256, task_target = 8192, jpar_target = 256;
     int wj4_max = 8192, wj2_max = 16384;
 };
 

@@ -210,9 +210,10 @@ long long hhr_steps_launched(const hhr_ctx *ctx);
 int hhr_set_step_limit(hhr_ctx *ctx, long long max_block_steps);
 /* Work split of the persistent force kernel (no reference counterpart):
  * active counts up to jpar_max use j-parallel groups of 2 i; larger ones
- * i-parallel groups of 256 with at most smax j-slices; a block step aims at
+ * i-parallel groups of 512 i (128 or 256 when tiles are split over waves,
+ * hhr_set_split_waves) with at most smax j-slices; a block step aims at
  * task_target i-parallel (jpar_target j-parallel) tasks over all clusters
- * (both at most 8192).  Defaults 64, 256, 4096, 256 (measured best for a
+ * (both at most 8192).  Defaults 64, 256, 8192, 256 (measured best for a
  * 6th-order N = 65536 Plummer sphere). */
 int hhr_set_plan(hhr_ctx *ctx, int jpar_max, int smax, int task_target, int jpar_target);
 /* i-parallel tasks of clusters with at most wj4_max (wj2_max) active
