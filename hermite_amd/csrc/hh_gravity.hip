@@ -741,6 +741,7 @@ static RArgs r_args(hhr_ctx *c)
     a.part = c->part;
     a.npad = c->npad;
     a.ntiles_total = c->ntiles;
+    a.nclu = c->nclu;
     a.steplog = c->timing ? c->steplog : nullptr;
     a.steplog_cap = c->steplog_cap;
     return a;

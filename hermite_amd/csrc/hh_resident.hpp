@@ -93,6 +93,7 @@ struct RArgs {
     double *__restrict__ part;
     int npad;                    // SoA stride = total padded slots
     int ntiles_total;
+    int nclu;                    // clusters (fixed for the context)
     int *__restrict__ steplog;   // optional: active count of step s at [s % steplog_cap]
     int steplog_cap;
 };
@@ -381,7 +382,7 @@ __global__ __launch_bounds__(NT) void r_plan_kernel(RArgs a)
     __shared__ double s_wsum[NW];
     __shared__ long long s_scan[NW];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int nclu = a.glob->nclu;
+    const int nclu = a.nclu;
     const int p = a.glob->step & 1;
     const bool exact = a.glob->exact != 0;
     const RPlan pl = a.glob->plan;
@@ -485,6 +486,32 @@ __global__ __launch_bounds__(NT) void r_plan_kernel(RArgs a)
     }
 }
 
+// The fields of a cluster record that a force task reads.
+struct RCluF {
+    double eps2;
+    long long pofs;
+    int off, n, ntiles, nact, mode, wj, nslice, tps, jchunk, task0, ipad;
+};
+
+__device__ __forceinline__ RCluF r_clu_fields(const RClu &c)
+{
+    RCluF f;
+    f.eps2 = c.eps2;
+    f.pofs = c.pofs;
+    f.off = c.off;
+    f.n = c.n;
+    f.ntiles = c.ntiles;
+    f.nact = c.nact;
+    f.mode = c.mode;
+    f.wj = c.wj;
+    f.nslice = c.nslice;
+    f.tps = c.tps;
+    f.jchunk = c.jchunk;
+    f.task0 = c.task0;
+    f.ipad = c.ipad;
+    return f;
+}
+
 // Clusters whose first task (first active-list position) the force (reduce)
 // kernel keeps in LDS for its lookups; larger ensembles search the cluster
 // records in global memory.
@@ -539,7 +566,11 @@ __device__ __forceinline__ void r_force_body(const RArgs &a)
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int npad = a.npad;
-    const int nclu = a.glob->nclu;
+    const int nclu = a.nclu;
+    // a single cluster's task fields are read together with the task count
+    // (one round trip to memory instead of two)
+    RCluF c0{};
+    if (nclu == 1) c0 = r_clu_fields(a.clu[0]);
     const int total = a.glob->total_tasks;
     const bool keys_lds = nclu > 1 && nclu <= R_CLU_LDS;
     if (keys_lds) {
@@ -553,8 +584,9 @@ __device__ __forceinline__ void r_force_body(const RArgs &a)
     // grid that is a multiple of 8 task t runs on XCD t % 8, so slices that
     // are a multiple of 8 apart share an L2.
     for (int task = blockIdx.x; task < total; task += gridDim.x) {
-        const int k = nclu == 1 ? 0 : keys_lds ? r_find_lds(s_key, nclu, task) : r_find_cluster(a.clu, nclu, task);
-        const RClu &c = a.clu[k];
+        const RCluF c = nclu == 1 ? c0
+                                  : r_clu_fields(a.clu[keys_lds ? r_find_lds(s_key, nclu, task)
+                                                                : r_find_cluster(a.clu, nclu, task)]);
         const int local = task - c.task0;
         const int grp = local / c.nslice, slice = local - grp * c.nslice;
         const double eps2 = c.eps2;
@@ -804,7 +836,7 @@ template <int ORDER>
 __global__ __launch_bounds__(NT) void r_reduce_correct_kernel(RArgs a)
 {
     __shared__ int s_key[R_CLU_LDS];
-    const int nclu = a.glob->nclu, total = a.glob->total_active;
+    const int nclu = a.nclu, total = a.glob->total_active;
     const bool wide = (long long)total * 64 <= (long long)gridDim.x * NT;
     if (blockIdx.x * (wide ? NT / 64 : NT / RL) >= total) return;  // whole workgroup idle this step
     const bool keys_lds = nclu > 1 && nclu <= R_CLU_LDS;
