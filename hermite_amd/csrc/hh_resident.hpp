@@ -227,9 +227,30 @@ __device__ void r_correct(const RPart &p, const double *f, double etapow, double
     }
 }
 
+// The fields of a cluster record that the reduction and corrector read.
+struct RCluR {
+    double etapow, dtlim, dtmin;
+    long long pofs;
+    int act0, nslice, ipad, off;
+};
+
+__device__ __forceinline__ RCluR r_clu_rfields(const RClu &c)
+{
+    RCluR f;
+    f.etapow = c.etapow;
+    f.dtlim = c.dtlim;
+    f.dtmin = c.dtmin;
+    f.pofs = c.pofs;
+    f.act0 = c.act0;
+    f.nslice = c.nslice;
+    f.ipad = c.ipad;
+    f.off = c.off;
+    return f;
+}
+
 // Correct particle i with force f, clamp dt to dtmin, refresh tn[i].
 template <int ORDER>
-__device__ double r_finish(const RArgs &a, const RClu &c, int i, const double *f)
+__device__ double r_finish(const RArgs &a, const RCluR &c, int i, const double *f)
 {
     const RPart p{a.rp, a.npad, i};
     r_correct<ORDER>(p, f, c.etapow, c.dtlim);
@@ -798,7 +819,7 @@ __device__ __forceinline__ double r_lane_tree(double x)
 
 template <int ORDER, int RLX>
 __device__ __forceinline__ void r_reduce_body(const RArgs &a, const int *s_key, bool keys_lds, int nclu, int total,
-                                              int nb, double &tacc, int &kacc)
+                                              int nb, const RCluR &c0, double &tacc, int &kacc)
 {
     constexpr int NC = f_words(ORDER);
     constexpr int PB = NT / RLX;  // particles per workgroup pass
@@ -807,8 +828,11 @@ __device__ __forceinline__ void r_reduce_body(const RArgs &a, const int *s_key, 
         const int pos = base + threadIdx.x / RLX;
         const bool live = pos < total;
         const int k = !live || nclu == 1 ? 0 : keys_lds ? r_find_lds(s_key, nclu, pos) : r_find_active(a.clu, nclu, pos);
-        const RClu &c = a.clu[k];
+        const RCluR c = nclu == 1 ? c0 : r_clu_rfields(a.clu[k]);
         const int ii = pos - c.act0;
+        // the particle's index, read before the slice sums so that its
+        // record can be read as soon as they are done
+        const int ip = live && l == 0 ? a.ilist[c.off + ii] : 0;
         double S[NC];
 #pragma unroll
         for (int q = 0; q < NC; q++) S[q] = 0.0;
@@ -823,7 +847,7 @@ __device__ __forceinline__ void r_reduce_body(const RArgs &a, const int *s_key, 
 #pragma unroll
         for (int q = 0; q < NC; q++) S[q] = r_lane_tree<RLX>(S[q]);
         if (live && l == 0) {
-            const double tn = r_finish<ORDER>(a, c, a.ilist[c.off + ii], S);
+            const double tn = r_finish<ORDER>(a, c, ip, S);
             if (kacc != k && kacc >= 0) atomicMin(&a.clu[kacc].tmin_slot[nb][blockIdx.x % R_SLOTS], r_bits(tacc));
             if (kacc != k) tacc = r_inf();
             kacc = k;
@@ -836,7 +860,11 @@ template <int ORDER>
 __global__ __launch_bounds__(NT) void r_reduce_correct_kernel(RArgs a)
 {
     __shared__ int s_key[R_CLU_LDS];
-    const int nclu = a.nclu, total = a.glob->total_active;
+    const int nclu = a.nclu;
+    // a single cluster's fields are read together with the step's counts
+    RCluR c0{};
+    if (nclu == 1) c0 = r_clu_rfields(a.clu[0]);
+    const int total = a.glob->total_active;
     const bool wide = (long long)total * 64 <= (long long)gridDim.x * NT;
     if (blockIdx.x * (wide ? NT / 64 : NT / RL) >= total) return;  // whole workgroup idle this step
     const bool keys_lds = nclu > 1 && nclu <= R_CLU_LDS;
@@ -849,8 +877,8 @@ __global__ __launch_bounds__(NT) void r_reduce_correct_kernel(RArgs a)
     // one cluster and flushes it when the cluster changes
     double tacc = r_inf();
     int kacc = -1;
-    if (wide) r_reduce_body<ORDER, 64>(a, s_key, keys_lds, nclu, total, nb, tacc, kacc);
-    else r_reduce_body<ORDER, RL>(a, s_key, keys_lds, nclu, total, nb, tacc, kacc);
+    if (wide) r_reduce_body<ORDER, 64>(a, s_key, keys_lds, nclu, total, nb, c0, tacc, kacc);
+    else r_reduce_body<ORDER, RL>(a, s_key, keys_lds, nclu, total, nb, c0, tacc, kacc);
     // one atomic per workgroup when all its lanes worked for one cluster
     __shared__ int s_k[NT / 64];
     __shared__ double s_t[NT / 64];
