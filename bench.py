@@ -16,7 +16,7 @@ device).  One step is one dt_max interval of integration (the reference's
 integrate_one_dtmax, SRC/nbodysystem.h:904-944): the timed region is
 steps x dt_max time units from t = 0, started and ended by a stream
 synchronisation inside hhr_run, with the block steps replayed from a
-hipGraph of 16 steps (--direct: launched directly on one stream).  Warmup
+hipGraph of 64 steps (--direct: launched directly on one stream).  Warmup
 intervals run on a separate context that is discarded, so the timed run
 starts from t = 0.
 
@@ -50,7 +50,7 @@ from hermite_amd import workloads as W  # noqa: E402
 
 METRIC = "interactions/s and Gflops (60/97/144 flop conv.), split-prec vs FP64 at stated |ΔE/E|"
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector peak
-BATCH = 16               # block steps per launch batch
+BATCH = 64               # block steps per launch batch (one hipGraph)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "r_force_kernel_6th_n65536_traffic.txt")
 
 # Kept for the tests and tools that import the default workload's constants.

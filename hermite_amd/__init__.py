@@ -469,7 +469,7 @@ class Resident:
     def __del__(self):
         self.close()
 
-    def run(self, tend, use_graph=True, batch=8):
+    def run(self, tend, use_graph=True, batch=64):
         rc = hip_lib().hhr_run(self.h, tend, int(use_graph), batch)
         if rc == -5:
             raise StepLimitReached(f"hhr_run: {hip_lib().hhr_last_error(self.h).decode()}")
