@@ -961,13 +961,13 @@ int hhr_create(int order, int variant, int device, int nclu, const int *ns, cons
     if ((e = hipMemcpy(c->clu, hc.data(), sizeof(RClu) * nclu, hipMemcpyHostToDevice)) != hipSuccess) return bail("upload", e);
     if ((e = hipMemcpy(c->glob, &g, sizeof(RGlob), hipMemcpyHostToDevice)) != hipSuccess) return bail("upload", e);
     if ((e = hipMemset(c->jpr, 0, sizeof(double) * (size_t)Q * tot)) != hipSuccess) return bail("memset", e);
-    // force: sixteen workgroups per CU, eight times the two resident at
-    // once, so that a step's tasks (up to the 8192 of the plan's target)
-    // mostly get a workgroup each and the dispatcher balances them as
-    // workgroups finish; measured faster than 4 per CU (C2 1 %, C4 4 %) and
-    // than 32.  reduce: four per CU (16 is slower)
+    // sixteen workgroups per CU, several times those resident at once, so
+    // that the dispatcher balances the work as workgroups finish: a step's
+    // force tasks (up to the 8192 of the plan's target) mostly get a
+    // workgroup each.  Measured faster than 4 per CU (C2 1-2 %, C4 4 %) and
+    // no slower than 32
     c->force_grid = prop.multiProcessorCount * 16;
-    c->reduce_grid = prop.multiProcessorCount * 4;
+    c->reduce_grid = prop.multiProcessorCount * 16;
     *out = c;
     return HH_OK;
 }

@@ -42,6 +42,9 @@ constexpr int R_GI = 2;          // i per group of the j-parallel task (half the
 __host__ __device__ constexpr int r_ni(int mode) { return mode == EXACT ? 1 : 2; }
 constexpr int R_TASK_MAX = 8192; // largest task target a plan may ask for
 constexpr int R_SLOTS = 32;      // words a block-time minimum is spread over
+// Steps with at most this many active particles sum each particle's slices
+// on a whole wave (one round of loads); larger ones on 8 lanes
+constexpr int R_WIDE_MAX = 4096;
 
 // Plan parameters (hhr_set_plan): active counts up to jpar_max use
 // j-parallel groups; otherwise i-parallel groups of NT with at most smax
@@ -865,7 +868,7 @@ __global__ __launch_bounds__(NT) void r_reduce_correct_kernel(RArgs a)
     RCluR c0{};
     if (nclu == 1) c0 = r_clu_rfields(a.clu[0]);
     const int total = a.glob->total_active;
-    const bool wide = (long long)total * 64 <= (long long)gridDim.x * NT;
+    const bool wide = total <= R_WIDE_MAX;
     if (blockIdx.x * (wide ? NT / 64 : NT / RL) >= total) return;  // whole workgroup idle this step
     const bool keys_lds = nclu > 1 && nclu <= R_CLU_LDS;
     if (keys_lds) {

@@ -217,8 +217,8 @@ def test_resident_dtmin_clamp():
 
 @pytest.mark.parametrize("n", [3000, 9000])
 def test_resident_fp64_reduce_paths(n):
-    """Both slice-sum widths (a wave per particle while a step's active
-    particles fit one wave each, 8 lanes beyond) against the EXACT run of
+    """Both slice-sum widths (a wave per particle for steps of at most 4096
+    active particles, 8 lanes beyond) against the EXACT run of
     the same cluster: the same schedule in the first steps and energies that
     agree to rounding."""
     order, eps2, eta, eta_s = 4, (4.0 / n) ** 2, 0.1, 0.01
