@@ -322,3 +322,23 @@ def test_plan_and_split_wave_settings_are_independent():
         r.close()
     assert out[0][0] == out[1][0] and np.array_equal(out[0][1], out[1][1])
     assert not np.array_equal(out[0][1], out[2][1])
+
+
+def test_large_ensemble_bitwise_vs_oracle_inplace():
+    """2100 clusters of 3 to 12 particles: more clusters than the plan
+    kernel has threads (several per thread) and than the force and reduce
+    kernels keep in LDS (their cluster lookups search the records in global
+    memory).  Every cluster's EXACT integration is bit-identical to the
+    oracle's in-place schedule of that cluster alone."""
+    order, eps2, eta, eta_s = 6, (1 / 32) ** 2, 0.4, 0.1
+    sizes = [3 + k % 10 for k in range(2100)]
+    systems = [initialised(order, n, 1000 + k, eps2, eta, eta_s, 1 / 16) for k, n in enumerate(sizes)]
+    print(f"resident ensemble of {len(sizes)} clusters", flush=True)
+    r = H.Resident(order, "exact", [(o.ptcl(), eps2, eta, 1 / 16, 0.0, 0.0) for o in systems])
+    r.run(1 / 16, use_graph=True, batch=16)
+    for k, o in enumerate(systems):
+        o.integrate_until_inplace(1 / 16)
+        st = r.stats(k)
+        assert (st["num_step"], st["num_bstep"], st["tsys"]) == (o.num_step, o.num_bstep, o.tsys), k
+        assert np.array_equal(r.particles(k), o.ptcl()), k
+    r.close()
