@@ -760,7 +760,8 @@ static hipError_t r_launch_step(hhr_ctx *c, int s)
     hipError_t e = r_mark(c, 4 * s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((r_compact_kernel<ORDER, MODE == EXACT>), dim3(c->ntiles), dim3(NT), 0, c->stream, a);
-    hipLaunchKernelGGL((r_plan_kernel<ORDER>), dim3(1), dim3(NT), 0, c->stream, a);
+    if (c->nclu > 1)  // one cluster: planned by the compaction's last workgroup
+        hipLaunchKernelGGL((r_plan_kernel<ORDER>), dim3(1), dim3(NT), 0, c->stream, a);
     if ((e = r_mark(c, 4 * s + 1)) != hipSuccess) return e;
     hipLaunchKernelGGL((r_force_kernel<ORDER, MODE>), dim3(c->force_grid), dim3(NT), 0, c->stream, a);
     if ((e = r_mark(c, 4 * s + 2)) != hipSuccess) return e;

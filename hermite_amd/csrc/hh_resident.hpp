@@ -1,7 +1,7 @@
 // GPU-resident block-timestep integration of one or many independent
 // clusters (ensembles): the whole block step of SRC/nbodysystem.h:692-765
-// runs on the device, so a step costs four kernel launches (capturable in a
-// hipGraph) and no host round trip.
+// runs on the device, so a step costs four kernel launches (three for a
+// single cluster; capturable in a hipGraph) and no host round trip.
 //
 // Per block step:
 //   r_compact_kernel  active list of every cluster (wave-aggregated append),
@@ -10,6 +10,8 @@
 //                     that stay inactive
 //   r_plan_kernel     per-cluster dtlim, step counters, kernel choice and
 //                     geometry, task table of the persistent force kernel
+//                     (for a single cluster the compaction's last workgroup
+//                     does this instead)
 //   r_force_kernel    persistent: a fixed grid walks the step's table of
 //                     (cluster, i-group, j-slice) tasks, writing partial forces
 //   r_reduce_correct_kernel
@@ -82,6 +84,7 @@ struct RGlob {
     int total_active;
     int step;      // block steps planned; step & 1 selects the current buffers
     RPlan plan;
+    unsigned int cdone[2];  // compaction workgroups done, parity buffered (one cluster)
 };
 
 struct RArgs {
@@ -290,9 +293,14 @@ __device__ __forceinline__ double r_tmin_wave(const RClu &c, int p)
     return r_wave_min(t);
 }
 
+template <int ORDER>
+__device__ void r_plan_body(const RArgs &a, bool atomic_counts);
+
 // One workgroup per 256-slot tile; a tile belongs to one cluster.  Times
 // are positive, and positive doubles order like their bit patterns, so the
-// minima are integer atomicMin on the bits.
+// minima are integer atomicMin on the bits.  With one cluster the last
+// workgroup to finish also plans the step (r_plan_body), which saves the
+// plan kernel's launch and its dependent reads.
 template <int ORDER, bool EX>
 __global__ __launch_bounds__(NT) void r_compact_kernel(RArgs a)
 {
@@ -350,6 +358,20 @@ __global__ __launch_bounds__(NT) void r_compact_kernel(RArgs a)
 #pragma unroll
         for (int w = 0; w < Q; w++) a.ipr[(size_t)w * a.npad + pos] = q[w];
     }
+    if (a.nclu == 1) {
+        __shared__ int s_last;
+        if (tid == 0) {
+            // this tile's append has been performed (its atomic returned)
+            // before the count of finished workgroups goes up
+            __builtin_amdgcn_s_waitcnt(0);
+            s_last = atomicAdd(&a.glob->cdone[p], 1u) == gridDim.x - 1;
+        }
+        __syncthreads();
+        if (s_last) {
+            if (tid == 0) a.glob->cdone[p] = 0;  // next used two steps on
+            r_plan_body<ORDER>(a, true);
+        }
+    }
 }
 
 // calc_dtlim (SRC/nbodysystem.h:652-661) with floor() for the int cast;
@@ -399,7 +421,7 @@ __device__ T r_block_exscan(T v, T *s_buf, T *total)
 // proportion to each cluster's nact * n work; task, active-list and partial
 // offsets are block scans in cluster order.
 template <int ORDER>
-__global__ __launch_bounds__(NT) void r_plan_kernel(RArgs a)
+__device__ void r_plan_body(const RArgs &a, bool atomic_counts)
 {
     constexpr int NC = f_words(ORDER);
     constexpr int NW = NT / 64;
@@ -418,7 +440,10 @@ __global__ __launch_bounds__(NT) void r_plan_kernel(RArgs a)
     double w = 0.0;
     for (int k = k0; k < k1; k++) {
         RClu &c = a.clu[k];
-        c.nact = c.acnt[p];
+        // counts appended by workgroups of the running launch are read as
+        // atomics (from the point where those atomics were performed)
+        c.nact = atomic_counts ? __hip_atomic_load(&c.acnt[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                               : c.acnt[p];
         c.acnt[p] = 0;
         c.mode = 0;
         c.ntask = 0;
@@ -508,6 +533,12 @@ __global__ __launch_bounds__(NT) void r_plan_kernel(RArgs a)
         if (a.steplog) a.steplog[a.glob->step % a.steplog_cap] = (int)tot_act;
         a.glob->step += 1;
     }
+}
+
+template <int ORDER>
+__global__ __launch_bounds__(NT) void r_plan_kernel(RArgs a)
+{
+    r_plan_body<ORDER>(a, false);
 }
 
 // The fields of a cluster record that a force task reads.
