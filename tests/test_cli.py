@@ -151,3 +151,20 @@ def test_hermite_run_fp64_close_to_reference(fix, tmp_path):
     assert [x[0] for x in rows] == [x[0] for x in ref]
     for a, b in zip(rows, ref):
         assert abs(float(a[7]) - float(b[7])) < 1e-7
+
+
+@pytest.mark.parametrize("order,eta,eta_s", [(4, "0.1", "0.01"), (6, "0.5", "0.1"), (8, "0.75", "0.1")])
+def test_runbench_parameters_are_the_reference_run_files(order, eta, eta_s, tmp_path):
+    """hermite_amd.runbench feeds hermite_run the parameters of
+    RUN/inp.{4th,6th,8th} (eta, eta_s; tend 2, dt_max 1/16, kt_for_eps 1,
+    log / snapshot / dump intervals 0.25 / 2 / 1, snapin inp.dat)."""
+    from hermite_amd import runbench
+
+    r = run_cli(runbench.params(order, "fp64"), tmp_path, ["--params"])
+    assert r.returncode == 0, r.stderr
+    echo = dict(ln.split(" : ", 1) for ln in (r.stdout + r.stderr).splitlines() if " : " in ln)
+    want = {"order": str(order), "eta": eta, "eta_s": eta_s, "tend": "2", "dtmax": "0.0625", "kt_for_eps": "1",
+            "log_interval": "0.25", "snap_interval": "2", "dump_interval": "1", "snapin": "inp.dat",
+            "variant": "fp64"}
+    for k, v in want.items():
+        assert echo.get(k) == v, (k, echo.get(k))
