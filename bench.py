@@ -26,8 +26,11 @@ event-timed pass of the same integration gives the force kernel's mean
 launch time for the roofline and the time split by active count.  Beside
 it: the split-precision kernel's integration outcome, both kernels'
 all-pairs force rate (the reference's "Init N square" evaluation), the split
-kernel's force error against the FP64 kernel, and the reference's own AVX2
-back end, built from its sources, timed on the host cores (cpu_baseline).
+kernel's force error against the FP64 kernel, single force calls at
+nact = 1, 4, ..., 16384 and N with predict_all on the initial state (device
+time per call; --micro runs only these, --no-micro skips them), and the
+reference's own AVX2 back end, built from its sources, timed on the host
+cores (cpu_baseline).
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -163,6 +166,49 @@ def all_pairs(order, P0, eps2):
     res["split_rel_error_vs_fp64"] = err
     res["split_over_fp64"] = res["split"]["interactions_per_s"] / res["fp64"]["interactions_per_s"]
     return res
+
+
+MICRO_NACT = (1, 4, 16, 64, 256, 1024, 4096, 16384)
+PR_WORDS = {4: 7, 6: 10, 8: 13}  # predicted j record (pos, vel[, acc[, jrk]]), doubles
+PEAK_HBM_GBS = 8000.0
+
+
+def micro(order, P0, eps2, budget_ms=300.0):
+    """Single force calls (calc_force_on_first_nact without the copy back:
+    force kernel + slice reduction, tuned geometry, inputs resident) on the
+    first nact particles for nact = 1, 4, ..., 16384 and N, both kernels,
+    and predict_all of every particle (no dirty slots): mean device time per
+    call from HIP events around repeated calls."""
+    n = len(P0)
+    rows = {}
+    pred = None
+    for variant in ("fp64", "split"):
+        g = H.Gravity(order, n, variant)
+        try:
+            g.set_jp_all(P0)
+            g.predict_all(0.0)
+            if variant == "fp64":
+                ms0 = g.time_predict(0.0, 1)
+                ms = g.time_predict(0.0, int(max(5, min(1000, budget_ms / max(ms0, 1e-3)))))
+                nbytes = float(n) * (H.JPSTRIDE[order] + PR_WORDS[order]) * 8
+                pred = {"ms": ms, "bytes": nbytes, "gbytes_per_s": nbytes / (ms * 1e-3) * 1e-9,
+                        "frac_hbm": nbytes / (ms * 1e-3) * 1e-9 / PEAK_HBM_GBS}
+                log(f"micro predict_all: {1e3 * ms:.1f} us, {pred['gbytes_per_s']:.0f} GB/s")
+            for nact in [x for x in MICRO_NACT if x < n] + [n]:
+                ms0 = g.time_force(nact, eps2, 1)
+                ms = g.time_force(nact, eps2, int(max(3, min(1000, budget_ms / max(ms0, 1e-3)))))
+                inter = float(nact) * n
+                tf = inter * H.FLOPS[order] / (ms * 1e-3) * 1e-12
+                r = rows.setdefault(nact, {"nact": nact})
+                r[f"{variant}_us"] = 1e3 * ms
+                r[f"{variant}_interactions_per_s"] = inter / (ms * 1e-3)
+                r[f"{variant}_frac"] = tf / PEAK_FP64_TFLOPS
+            log(f"micro force {variant}: " + ", ".join(f"{k}: {r[variant + '_us']:.1f} us" for k, r in rows.items()))
+        finally:
+            g.close()
+    return {"predict_all": pred, "force": list(rows.values()),
+            "note": "device time per call; frac = conventional flops / 78.6 TF/s (unpacked FP32 issues at "
+                    "the FP64 rate, so the split kernel has the same peak)"}
 
 
 def traffic_from_profile(w):
@@ -303,6 +349,10 @@ def main():
                     help="time the event-timed pass itself instead of a separate untimed run (default for the "
                          "long workloads c3 and c5; the events add about 10 us per block step)")
     ap.add_argument("--no-split", action="store_true", help="skip the split-precision legs")
+    ap.add_argument("--no-micro", action="store_true", help="skip the single-force-call microbenchmark")
+    ap.add_argument("--micro", action="store_true",
+                    help="only the microbenchmark: predict_all and single force calls at nact = 1 ... N on the "
+                         "workload's initial state (one JSON line; single-cluster workloads)")
     ap.add_argument("--calibrate", action="store_true",
                     help="first run 5 device-to-device copies of 256 MiB (the reference point that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes)")
@@ -328,6 +378,18 @@ def main():
     cpu_tend = args.cpu_tend or w.dtmax
     log(f"workload {w.name}: init_step on the GPU ({len(w.clusters)} cluster(s), N = {w.n_total})")
     states = initial_states(w)
+    if args.micro:
+        if len(states) != 1:
+            raise SystemExit("--micro takes a single-cluster workload (c1, c2, c3, c5)")
+        m = micro(w.order, states[0][0], states[0][1])
+        big = m["force"][-1]
+        print(json.dumps({"metric": "single force calls and predict_all by active count", "value": big["fp64_interactions_per_s"],
+                          "unit": "interactions/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f64",
+                          "data": "synthetic (seeded Plummer spheres)",
+                          "config": {"workload": f"{w.name}: initial state after init_step, N = {w.n_total}",
+                                     "name": w.name, "order": w.order, "n": w.n_total},
+                          "micro": m}), flush=True)
+        return
     if args.warmup > 0:
         log(f"warmup: {args.warmup} interval(s) on a discarded context")
         integrate(w, "fp64", states, min(args.warmup, steps) * w.dtmax, graph=graph)
@@ -367,6 +429,9 @@ def main():
             split.update({"all_pairs_interactions_per_s": ap2["split"]["interactions_per_s"],
                           "all_pairs_over_fp64": ap2["split_over_fp64"],
                           "rel_error_vs_fp64": ap2["split_rel_error_vs_fp64"]})
+    mic = None
+    if not args.no_micro and len(states) == 1:
+        mic = micro(w.order, states[0][0], states[0][1])
     cpu = None
     if not args.no_cpu:
         log(f"CPU baseline: reference AVX2 back end (t=0->{cpu_tend:g} where it integrates)")
@@ -403,6 +468,7 @@ def main():
         "all_pairs_fp64": None if ap2 is None else {
             "interactions_per_s": ap2["fp64"]["interactions_per_s"], "gflops": ap2["fp64"]["gflops"],
             "force_ms": ap2["fp64"]["force_ms"], "frac": ap2["fp64"]["gflops"] * 1e-3 / PEAK_FP64_TFLOPS},
+        "micro": mic,
         "cpu_baseline": cpu,
     }
     line["timing"] = "one event-timed pass" if single else "untimed run, then an event-timed pass for the roofline"
