@@ -303,8 +303,16 @@ def cpu_baseline(w, states, cpu_tend, budget_s=20.0):
         sample = (f"reference driver + AVX2 back end on the first {done} clusters of the ensemble whose N_k is "
                   f"a multiple of 4, each t=0->{cpu_tend:g}" + (f" run {reps} times" if reps > 1 else "")
                   + f" ({secs:.1f} s), {cpu}")
-    return {"value": inter / secs, "unit": "interactions/s", "cores": used, "kind": "reference",
-            "sample": sample, "gflops": inter / secs * H.FLOPS[w.order] * 1e-9}
+    out = {"value": inter / secs, "unit": "interactions/s", "cores": used, "kind": "reference",
+           "sample": sample, "gflops": inter / secs * H.FLOPS[w.order] * 1e-9}
+    if len(states) == 1 and len(states[0][0]) <= 65536 and len(states[0][0]) % 4 == 0:
+        # the reference's "Init N square" evaluation: all N on all N, once
+        P0, eps2 = states[0][0], states[0][1]
+        n = len(P0)
+        _, ai = ref.forces(P0, 0.0, n, eps2, reps=1, threads=threads, want_output=False)
+        out["all_pairs"] = {"interactions_per_s": float(n) * n / ai["seconds"], "seconds": ai["seconds"],
+                            "gflops": float(n) * n * H.FLOPS[w.order] / ai["seconds"] * 1e-9}
+    return out
 
 
 def by_active_count(log_, n_j):
