@@ -18,6 +18,7 @@ hh_load_tuning_file.
 
     python -m hermite_amd.autotune --orders 4 6 8 --variants fp64 split \\
         --sizes 1024 4096 16384 65536 262144
+    python -m hermite_amd.autotune --orders 6 --sizes 1048576 --max-nact 16384
 """
 import argparse
 import json
@@ -57,9 +58,10 @@ def _state(order, nbody, seed=7):
     return P
 
 
-def tune(order, variant, nbody, reps=None, device=0, log=None):
-    """Measure the candidates at active counts 1, 2, 4, ..., nbody and return
-    (rows, measurements)."""
+def tune(order, variant, nbody, reps=None, device=0, log=None, max_nact=0):
+    """Measure the candidates at active counts 1, 2, 4, ..., nbody (up to
+    max_nact when it is set and below nbody; larger counts then keep the
+    built-in geometry) and return (rows, measurements)."""
     import hermite_amd as H
     eps2 = (4.0 / nbody) ** 2
     g = H.Gravity(order, nbody, variant, device=device, tuning=False)
@@ -67,6 +69,8 @@ def tune(order, variant, nbody, reps=None, device=0, log=None):
         g.set_jp_all(_state(order, nbody))
         g.predict_all(1e-3)
         nacts = sorted(set([1 << k for k in range(int(math.log2(nbody)) + 1)] + [nbody]))
+        if 0 < max_nact < nbody:
+            nacts = [x for x in nacts if x <= max_nact]
         best, meas = [], {}
         for nact in nacts:
             work = nact * nbody
@@ -89,7 +93,10 @@ def tune(order, variant, nbody, reps=None, device=0, log=None):
         g.close()
     rows = []
     for q, (nact, k, ni, s, _) in enumerate(best):
-        hi = INT_MAX if q + 1 == len(best) else int(math.sqrt(nact * best[q + 1][0]))
+        if q + 1 < len(best):
+            hi = int(math.sqrt(nact * best[q + 1][0]))
+        else:
+            hi = INT_MAX if nact == nbody else nact
         if rows and rows[-1][1:] == (k, ni, s):
             rows[-1] = (hi, k, ni, s)
         else:
@@ -140,6 +147,8 @@ def main(argv=None):
     ap.add_argument("--out", default=DEFAULT_PATH)
     ap.add_argument("--log", default=None, help="also write every measurement here (JSON)")
     ap.add_argument("--device", type=int, default=0)
+    ap.add_argument("--max-nact", type=int, default=0,
+                    help="largest active count measured (0: N); above it the built-in geometry applies")
     a = ap.parse_args(argv)
     tables = load(a.out) if os.path.exists(a.out) else {}
     allmeas = {}
@@ -147,7 +156,7 @@ def main(argv=None):
     for order in a.orders:
         for variant in a.variants:
             for n in a.sizes:
-                rows, meas = tune(order, variant, n, device=a.device, log=sys.stdout)
+                rows, meas = tune(order, variant, n, device=a.device, log=sys.stdout, max_nact=a.max_nact)
                 tables[key(order, variant, n)] = [list(r) for r in rows]
                 allmeas[key(order, variant, n)] = meas
                 print(f"{key(order, variant, n)}: {rows}  ({time.time() - t0:.0f} s)", flush=True)
