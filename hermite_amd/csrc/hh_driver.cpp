@@ -933,6 +933,24 @@ int hhd_set_restart_state(hhd_sys *s, double e0, double eprev, long nstep_tot, l
     return HH_OK;
 }
 
+// Records at a dt_max boundary from another integrator: every tlast equals
+// tsys, so grouping by dt level orders the next block times as the driver
+// expects (the state integrate_one_dtmax leaves).
+int hhd_load_state(hhd_sys *s, const double *recs, double tsys, long num_step, long num_bstep, long dtmin_hits)
+{
+    if (!s || !s->g || !recs) return HH_E_ARG;
+    for (int i = 0; i < s->n; i++)
+        if (recs[(size_t)i * s->ps + TLAST] != tsys) return HH_E_ARG;
+    std::memcpy(s->ptcl.data(), recs, sizeof(double) * (size_t)s->n * s->ps);
+    s->tsys = tsys;
+    s->num_step = num_step;
+    s->num_bstep = num_bstep;
+    s->dtmin_hits = dtmin_hits;
+    s->sort_by_level(s->n, s->dtmax);
+    for (int i = 0; i < s->n; i++) s->set_jp(i);
+    return s->rc;
+}
+
 int hhd_reinitialize_steps(hhd_sys *s, double eta, double dtmax)
 {
     if (!s || !s->g) return HH_E_STATE;

@@ -16,6 +16,13 @@
 //   variant fp64|split|exact   force kernel (default fp64)
 //   device  <ordinal>          HIP device (default 0)
 //   dtmin <dt> | dtmininv <1/dt>   lower clamp on the timestep (default none)
+//   resident 0|1               1: integrate with the GPU-resident block step
+//                              (hhr_*) between output times instead of the
+//                              host driver (orders 4, 6, 8; default 0).  The
+//                              outputs keep their formats; the particles are
+//                              not permuted between block steps, so the j
+//                              summation order, and with it the last bits of
+//                              a run, differ from the host driver's.
 // The order comes from the "order" tag (4, 6 or 8; "8dd" selects the
 // double-double 8th-order build, which the reference selects at compile
 // time with -DEIGHTHDD and whose parser reads "8dd" as 8); the reference
@@ -28,6 +35,7 @@
 
 #include <omp.h>
 
+#include <algorithm>
 #include <cctype>
 #include <cmath>
 #include <cstdio>
@@ -49,6 +57,7 @@ struct Params {
     double log_interval, snap_interval, dump_interval;
     std::string variant = "fp64";
     int device = 0;
+    int resident = 0;
     double dtmin = 0.0;
 
     Params()
@@ -96,6 +105,7 @@ struct Params {
             else if (tag == "variant") ss >> variant;
             else if (tag == "device") ss >> device;
             else if (tag == "dtmin") ss >> dtmin;
+            else if (tag == "resident") ss >> resident;
             else if (tag == "dtmininv") inv(dtmin);
             else std::cerr << "unmatched tag : " << tag << std::endl;
         }
@@ -115,7 +125,8 @@ struct Params {
 
     void print_extra(std::ostream &os) const
     {
-        os << "variant : " << variant << "\n" << "device : " << device << "\n" << "dtmin : " << dtmin << std::endl;
+        os << "variant : " << variant << "\n" << "device : " << device << "\n" << "dtmin : " << dtmin << "\n"
+           << "resident : " << resident << std::endl;
     }
 
     // Parameter::assert_check (SRC/parameter.h:56-65); returns the failed
@@ -133,6 +144,8 @@ struct Params {
         if (variant != "fp64" && variant != "split" && variant != "exact") return "variant must be fp64, split or exact";
         if (!(dtmin >= 0.0)) return "dtmin >= 0";
         if (dd && variant == "split") return "order 8dd has no split variant";
+        if (resident != 0 && resident != 1) return "resident must be 0 or 1";
+        if (resident && dd) return "order 8dd has no GPU-resident driver";
         return "";
     }
 };
@@ -178,6 +191,7 @@ struct Prof {
     enum { NPH = 9, TOTAL = 8 };
     double base[NPH] = {0}, total = 0.0, tbeg = 0.0;
     long ns_prev = 0, nbs_prev = 0;
+    bool resident = false;  // no host predict phase: no bandwidth line
     void flush(hhd_sys *s)
     {
         hhd_profile(s, base);
@@ -207,7 +221,7 @@ struct Prof {
         std::fprintf(stdout, "## %ld %e %e %e %e\n", nbody, total, gflops, total / nbs * 1.e6, nact);
         const double nread = double(nbs) * nbody * 8.0 * hh_jp_words(order);
         const double nwrite = double(nbs) * nbody * 8.0 * hh_pred_words(order);
-        std::fprintf(stdout, "prediction bandwidth = %f GB/s\n", (nread + nwrite) / t[2] * 1.e-9);
+        if (!resident) std::fprintf(stdout, "prediction bandwidth = %f GB/s\n", (nread + nwrite) / t[2] * 1.e-9);
         std::fflush(stderr);
         std::fflush(stdout);
     }
@@ -302,17 +316,59 @@ int main(int argc, char **argv)
         if (hhd_reinitialize_steps(s, prm.eta, prm.dtmax) != HH_OK) return die(s, "reinitialize_steps");
     }
 
+    // GPU-resident mode: the block steps run in hhr_* from the state the
+    // host driver has initialised; the records come back at each output time
+    hhr_ctx *r = nullptr;
+    std::vector<double> rbuf;
+    long ns0 = 0, nbs0 = 0, hits0 = 0;
+    if (prm.resident) {
+        rbuf.resize((size_t)n * pwords(order));
+        hhd_get_ptcl(s, rbuf.data());
+        const double par[5] = {hhd_eps2(s), prm.eta, prm.dtmax, prm.dtmin, hhd_tsys(s)};
+        const int nn = (int)n;
+        if (hhr_create(order, variant_code(prm.variant), prm.device, 1, &nn, rbuf.data(), par, &r) != HH_OK)
+            return die(s, std::string("hhr_create: ") + hhr_last_error(nullptr));
+        ns0 = hhd_num_step(s);
+        nbs0 = hhd_num_bstep(s);
+        hits0 = hhd_dtmin_hits(s);
+        std::fprintf(stderr, "GPU-resident block steps\n");
+    }
+    auto fail_run = [&](const std::string &what) {
+        std::string msg = what;
+        if (r) {
+            const char *e = hhr_last_error(r);
+            if (e && *e) msg += std::string(": ") + e;
+            hhr_destroy(r);
+        }
+        return die(s, msg);
+    };
+
     double tnext_log = hhd_tsys(s) + prm.log_interval;
     double tnext_snap = hhd_tsys(s) + prm.snap_interval;
     double tnext_dump = hhd_tsys(s) + prm.dump_interval;
     double wt_log = omp_get_wtime();
     Prof prof;
+    prof.resident = r != nullptr;
     prof.ns_prev = hhd_num_step_tot(s);
     prof.nbs_prev = hhd_num_bstep_tot(s);
     prof.flush(s);
     prof.beg();
     while (hhd_tsys(s) < prm.tend) {
-        if (hhd_integrate_one_dtmax(s) != HH_OK) return die(s, "integrate_one_dtmax");
+        if (r) {
+            // whole dt_max intervals up to the next output time
+            const double t0 = hhd_tsys(s);
+            const double tev = std::min(std::min(tnext_log, tnext_snap), std::min(tnext_dump, prm.tend));
+            const double target = t0 + std::max(1.0, std::ceil((tev - t0) / prm.dtmax)) * prm.dtmax;
+            if (hhr_run(r, target, 1, 64) != HH_OK) return fail_run("hhr_run");
+            long long ns = 0, nbs = 0, hits = 0;
+            double tr = 0.0;
+            if (hhr_stats(r, 0, &ns, &nbs, &tr, &hits) != HH_OK || hhr_get_particles(r, 0, rbuf.data()) != HH_OK)
+                return fail_run("hhr_stats");
+            if (hhd_load_state(s, rbuf.data(), tr, ns0 + (long)ns, nbs0 + (long)nbs, hits0 + (long)hits) != HH_OK)
+                return fail_run("hhd_load_state");
+        } else if (hhd_integrate_one_dtmax(s) != HH_OK) {
+            return die(s, "integrate_one_dtmax");
+        }
         const double t = hhd_tsys(s);
         if (t >= tnext_log) {
             tnext_log += prm.log_interval;
@@ -325,9 +381,9 @@ int main(int argc, char **argv)
             snapid++;
             if (prm.snapout_base.length() > 0) {
                 std::string fname;
-                if (!format_name(prm.snapout_base, snapid, fname)) return die(s, "bad snapout_base pattern");
+                if (!format_name(prm.snapout_base, snapid, fname)) return fail_run("bad snapout_base pattern");
                 std::fprintf(stderr, "snap file : %s\n", fname.c_str());
-                if (hhd_write_snapshot(s, fname.c_str(), snapid) != HH_OK) return die(s, hh_io_error());
+                if (hhd_write_snapshot(s, fname.c_str(), snapid) != HH_OK) return fail_run(hh_io_error());
             }
         }
         if (t >= tnext_dump) {
@@ -337,14 +393,15 @@ int main(int argc, char **argv)
             prof.show(s, n, order);
             if (prm.dumpout_base.length() > 0) {
                 std::string fname;
-                if (!format_name(prm.dumpout_base, dumpid, fname)) return die(s, "bad dumpout_base pattern");
+                if (!format_name(prm.dumpout_base, dumpid, fname)) return fail_run("bad dumpout_base pattern");
                 std::fprintf(stderr, "dump file : %s\n", fname.c_str());
-                if (hhd_dump(s, fname.c_str(), dumpid) != HH_OK) return die(s, hh_io_error());
+                if (hhd_dump(s, fname.c_str(), dumpid) != HH_OK) return fail_run(hh_io_error());
             }
             prof.flush(s);
             prof.beg();
         }
     }
+    if (r) hhr_destroy(r);
     hhd_destroy(s);
     return 0;
 }

@@ -5,11 +5,16 @@ square" rate and, per N-body time unit, wall time, Gflops (num_step * N *
 60/97/144 over the wall time, counting everything, SRC/nbodysystem.h:147),
 microseconds per block step and the mean active count.
 
-    python -m hermite_amd.runbench --order 6 --n 65536 [--variant fp64]
+    python -m hermite_amd.runbench --order 6 --n 65536 [--variant fp64] [--resident]
+
+--resident runs the block steps GPU-resident between output times (the
+"resident 1" tag); the particles then keep their order, so the j summation
+order and the last bits of the run differ from the reference's.
 
 The input is generated as RUN/gen-plum.c does (seed 19640916, radii below
-100), and the parameters are those of RUN/inp.{4th,6th,8th}; the mean active
-counts therefore match the reference's logs (RUN/gpu-*.log) line for line.
+100), and the parameters are those of RUN/inp.{4th,6th,8th}; with the host
+driver the mean active counts therefore match the reference's logs
+(RUN/gpu-*.log) line for line.
 """
 import argparse
 import os
@@ -26,12 +31,12 @@ from . import make_particles, plummer, write_snapshot
 _ETA = {4: (0.1, 0.01), 6: (0.5, 0.1), 8: (0.75, 0.1)}
 
 
-def params(order: int, variant: str) -> str:
+def params(order: int, variant: str, resident: bool = False) -> str:
     eta, eta_s = _ETA[order]
     return (f"order {order}\neta   {eta}\neta_s {eta_s}\n\nsnapin     inp.dat\n\n"
             "tend       2.0\ndtmaxinv   16\nkt_for_eps 1.0\n\n"
             "log_interval  0.25\nsnap_interval 2.0\ndump_interval 1.0\n"
-            f"variant {variant}\n")
+            f"variant {variant}\n" + ("resident 1\n" if resident else ""))
 
 
 def main(argv=None) -> int:
@@ -39,6 +44,7 @@ def main(argv=None) -> int:
     ap.add_argument("--order", type=int, default=6, choices=[4, 6, 8])
     ap.add_argument("--n", type=int, default=65536)
     ap.add_argument("--variant", default="fp64", choices=["fp64", "split", "exact"])
+    ap.add_argument("--resident", action="store_true", help="GPU-resident block steps (tag resident 1)")
     args = ap.parse_args(argv)
     exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "hermite_run")
     if not os.path.exists(exe):
@@ -48,7 +54,7 @@ def main(argv=None) -> int:
     with tempfile.TemporaryDirectory() as wd:
         write_snapshot(os.path.join(wd, "inp.dat"), args.order, 0, 0.0, P, np.zeros(args.n))
         t0 = time.time()
-        out = subprocess.run([exe], input=params(args.order, args.variant), text=True, capture_output=True, cwd=wd)
+        out = subprocess.run([exe], input=params(args.order, args.variant, args.resident), text=True, capture_output=True, cwd=wd)
     if out.returncode:
         sys.stderr.write(out.stderr)
         return out.returncode

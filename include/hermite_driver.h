@@ -74,6 +74,14 @@ double hhd_eps2(const hhd_sys *s);
 int hhd_reinitialize_steps(hhd_sys *s, double eta, double dtmax);
 /* Energies and step totals a dump carries (restore, SRC/nbodysystem.h:428-446). */
 int hhd_set_restart_state(hhd_sys *s, double e0, double eprev, long nstep_tot, long nbstep_tot);
+/* State integrated elsewhere, at a dt_max boundary (no reference counterpart:
+ * for hermite_run's GPU-resident mode, which integrates with hhr_* and hands
+ * the records back for energies, logs, snapshots and dumps).  recs: n
+ * Particle records, every one with tlast == tsys; num_step, num_bstep and
+ * dtmin_hits replace the counters since hhd_create.  The dt-level partition
+ * and the j-store are rebuilt, so host integration may continue.  HH_E_ARG
+ * when a record is not at tsys. */
+int hhd_load_state(hhd_sys *s, const double *recs, double tsys, long num_step, long num_bstep, long dtmin_hits);
 /* print_stat (SRC/nbodysystem.h:934-969) without the printing: out[9] =
  * tsys, block steps and particle steps since the previous call, their
  * totals, mean nact, dE local, dE global, energy. */

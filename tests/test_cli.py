@@ -168,3 +168,45 @@ def test_runbench_parameters_are_the_reference_run_files(order, eta, eta_s, tmp_
             "variant": "fp64"}
     for k, v in want.items():
         assert echo.get(k) == v, (k, echo.get(k))
+
+
+def _stat_rows(stdout):
+    return [ln.split() for ln in stdout.splitlines() if re.match(r"^\d+\.\d+ ", ln)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("order", [4, 6, 8])
+def test_hermite_run_resident_mode(fix, order, tmp_path):
+    """resident 1: block steps GPU-resident between output times.  The
+    particles are not permuted between block steps, so the j summation order,
+    and with it the last bits, differ from the reference's run: the same
+    output times and files, the reference's energy errors to 1e-7, every
+    particle in every snapshot and dump (the dumps' records all at the
+    output time); a restart
+    from the resident run's dump runs resident to the end."""
+    case = fix["cases"][str(order)]
+    (tmp_path / "snap0.dat").write_text(case["snap0"])
+    print(f"hermite_run order {order} resident", flush=True)
+    r = run_cli(case["inp"] + "variant exact\nresident 1\n", tmp_path)
+    assert r.returncode == 0, r.stderr[-2000:]
+    rows, ref = _stat_rows(r.stdout), _stat_rows(case["stdout"])
+    assert [x[0] for x in rows] == [x[0] for x in ref]
+    for a, b in zip(rows, ref):
+        assert abs(float(a[7]) - float(b[7])) < 1e-7
+        assert abs(float(a[6]) - float(b[6])) < 1e-7
+    _, _, P0 = H.read_snapshot(str(tmp_path / "snap0.dat"), order)
+    for name in case["files"]:
+        path = str(tmp_path / name)
+        if "snap" in name:
+            _, t, P = H.read_snapshot(path, order)
+        else:
+            d = H.read_dump(path, order)
+            t, P = d["tsys"], d["P"]
+            assert np.all(P[:, 2] == t), name  # every tlast at the output time
+        assert sorted(P[:, 0]) == sorted(P0[:, 0]), name
+        assert np.all(np.isfinite(P)), name
+    (tmp_path / "dump_in.dat").write_text((tmp_path / "out_dump001.dat").read_text())
+    r2 = run_cli(case["inp_restart"] + "variant exact\nresident 1\n", tmp_path)
+    assert r2.returncode == 0, r2.stderr[-2000:]
+    rows2, ref2 = _stat_rows(r2.stdout), _stat_rows(case["stdout_restart"])
+    assert [x[0] for x in rows2] == [x[0] for x in ref2]
