@@ -53,6 +53,7 @@ from hermite_amd import workloads as W  # noqa: E402
 
 METRIC = "interactions/s and Gflops (60/97/144 flop conv.), split-prec vs FP64 at stated |ΔE/E|"
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector peak
+PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector peak
 BATCH = 64               # block steps per launch batch (one hipGraph)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "r_force_kernel_6th_n65536_traffic.txt")
 
@@ -91,42 +92,60 @@ def initial_states(w):
     return out
 
 
-def integrate(w, variant, states, tend, timing=False, graph=False, chunks=1, wall_limit=None, max_bsteps=0):
+def integrate(w, variant, states, tend, timing=False, graph=False, chunks=1, wall_limit=None, max_bsteps=0,
+              t_energy=None):
     """Resident integration t = 0 -> tend, in `chunks` equal time spans
     (each hhr_run call ends with a stream synchronisation; a progress line
-    per chunk).  With wall_limit (s) the run stops after the first chunk
-    that ends past it; with max_bsteps a chunk stops after that many block
-    steps."""
+    per chunk).  wall_s counts the hhr_run calls only.  With t_energy
+    (0 < t_energy < tend, a chunk boundary after rounding up) |dE/E| is also
+    recorded at that time, between two hhr_run calls.  With wall_limit (s)
+    the run stops after the first chunk that ends past it; with max_bsteps a
+    chunk stops after that many block steps."""
     clus = [(P0, eps2, w.eta, w.dtmax, w.dtmin, 0.0) for P0, eps2, _, _ in states]
     r = H.Resident(w.order, variant, clus)
+
+    def de_now():
+        return np.array([abs((r.energy(k) - states[k][2]) / states[k][2]) for k in range(len(states))])
+
     try:
         if timing:
             r.set_timing(True)
         if max_bsteps:
             r.set_step_limit(max_bsteps)
-        t0 = time.perf_counter()
-        err = None
+        ends = [tend * k / chunks for k in range(1, chunks + 1)]
+        if t_energy is not None and 0 < t_energy < tend and t_energy not in ends:
+            ends = sorted(ends + [t_energy])
+        wall, err, de_at = 0.0, None, None
         try:
-            for k in range(1, chunks + 1):
-                r.run(tend * k / chunks, use_graph=graph and not timing, batch=BATCH)
-                if chunks > 1:
-                    log(f"  {variant} t={tend * k / chunks:g}: {time.perf_counter() - t0:.1f} s")
-                if wall_limit is not None and k < chunks and time.perf_counter() - t0 > wall_limit:
+            for t in ends:
+                t0 = time.perf_counter()
+                r.run(t, use_graph=graph and not timing, batch=BATCH)
+                wall += time.perf_counter() - t0
+                if len(ends) > 1:
+                    log(f"  {variant} t={t:g}: {wall:.1f} s")
+                if t == t_energy:
+                    de_at = de_now()
+                if wall_limit is not None and t < tend and wall > wall_limit:
                     err = f"stopped by the benchmark after {wall_limit:g} s of wall time"
                     break
         except H.HHError as ex:
             err = str(ex)
-        wall = time.perf_counter() - t0
         st = [r.stats(k) for k in range(len(states))]
         inter = float(sum(s["num_step"] * len(states[k][0]) for k, s in enumerate(st)))
         out = {"wall_s": wall, "interactions": inter, "num_step": sum(s["num_step"] for s in st),
                "num_bstep": max(s["num_bstep"] for s in st), "tsys": min(s["tsys"] for s in st),
                "dtmin_hits": sum(s["dtmin_hits"] for s in st), "error": err}
         if err is None:
-            de = np.array([abs((r.energy(k) - states[k][2]) / states[k][2]) for k in range(len(states))])
+            de = de_now()
             out["dE_E"] = float(de.max())
             if len(de) > 1:
                 out["dE_E_median"] = float(np.median(de))
+            if t_energy is not None and t_energy >= tend:
+                de_at = de
+        if de_at is not None:
+            out["dE_E_at_t_energy"] = float(de_at.max())
+            if len(de_at) > 1:
+                out["dE_E_median_at_t_energy"] = float(np.median(de_at))
         if timing:
             out["force_ms"], out["force_launches"] = r.force_timing()
             out["log"] = r.step_log()
@@ -328,6 +347,46 @@ def by_active_count(log_, n_j):
     return out
 
 
+def kernel_rate(ft, order):
+    """Mean event-timed force launch (ms) and the conventional TF/s of a
+    launch (interactions x 60/97/144 flops per launch / its mean time)."""
+    launches = max(ft["force_launches"], 1)
+    avg_ms = ft["force_ms"] / launches
+    flops_per_launch = ft["interactions"] * H.FLOPS[order] / launches
+    return avg_ms, flops_per_launch / (avg_ms * 1e-3) * 1e-12
+
+
+def opmix_from_profile(order):
+    """FP64 and FP32 shares of the split force kernel's VALU arithmetic
+    instructions, from the rocprofv3 counter summary of this order, if
+    recorded: {"f64": share, "f32": share}."""
+    path = os.path.join(ROOT, "profiles", f"r_force_kernel_{order}th_split_opmix.md")
+    try:
+        with open(path) as fh:
+            for ln in fh:
+                if ln.startswith("opmix:"):
+                    return json.loads(ln.split(":", 1)[1])
+    except (OSError, ValueError):
+        pass
+    return None
+
+
+def split_roofline(w, avg_ms, achieved, launches):
+    """Roofline of r_force_kernel<ORDER,SPLIT>: its conventional TF/s against
+    the FP64 vector peak, the FP32 vector peak, and the blend of the two that
+    its measured instruction mix allows (1 / (s64/78.6 + s32/157.3) for
+    FP64 and FP32 VALU instruction shares s64, s32)."""
+    mix = opmix_from_profile(w.order)
+    out = {"bound": "valu", "kernel": f"r_force_kernel<{w.order},SPLIT>", "achieved": achieved,
+           "unit": "TFLOP/s", "avg_launch_ms": avg_ms, "launches": launches,
+           "peak_fp64": PEAK_FP64_TFLOPS, "frac_fp64": achieved / PEAK_FP64_TFLOPS,
+           "peak_fp32": PEAK_FP32_TFLOPS, "frac_fp32": achieved / PEAK_FP32_TFLOPS, "opmix": mix}
+    if mix:
+        blend = 1.0 / (mix["f64"] / PEAK_FP64_TFLOPS + mix["f32"] / PEAK_FP32_TFLOPS)
+        out.update({"peak_blended": blend, "frac_blended": achieved / blend})
+    return out
+
+
 def workload_text(w, tend, graph):
     kind = "hipGraph replay" if graph else "direct launches on one stream"
     if w.name == "c4":
@@ -403,8 +462,9 @@ def main():
         integrate(w, "fp64", states, min(args.warmup, steps) * w.dtmax, graph=graph)
     single = args.single_pass or w.name in ("c3", "c5")
     chunks = 16 if w.name in ("c3", "c5") else 1
+    t_cfg = w.tend  # the workload's own end time: |dE/E| is reported there too
     log(f"timed: FP64 resident integration t=0->{tend:g}" + (" (event-timed)" if single else ""))
-    f = integrate(w, "fp64", states, tend, timing=single, graph=graph, chunks=chunks)
+    f = integrate(w, "fp64", states, tend, timing=single, graph=graph, chunks=chunks, t_energy=t_cfg)
     if f["error"]:
         raise SystemExit(f"FP64 integration failed: {f['error']}")
     value = f["interactions"] / f["wall_s"]
@@ -414,24 +474,28 @@ def main():
     else:
         log("event-timed pass of the same integration (roofline)")
         ft = integrate(w, "fp64", states, tend, timing=True)
-    launches = max(ft["force_launches"], 1)
-    avg_ms = ft["force_ms"] / launches
-    flops_per_launch = ft["interactions"] * H.FLOPS[w.order] / launches
-    achieved = flops_per_launch / (avg_ms * 1e-3) * 1e-12
-    split, ap2 = None, None
+    avg_ms, achieved = kernel_rate(ft, w.order)
+    split, ap2, split_roof = None, None, None
     if not args.no_split:
-        # 6th and 8th order do not integrate with the FP32 body (step
-        # collapse, or steps pinned at dt_min when there is a clamp): one
-        # interval at most, in small chunks, for at most 30 s and 1024
-        # block steps
-        split_tend = tend if w.order == 4 else w.dtmax
-        log(f"split-precision integration t=0->{split_tend:g}")
-        s = integrate(w, "split", states, split_tend, chunks=chunks if w.order == 4 else 256,
-                      wall_limit=None if w.order == 4 else 30.0, max_bsteps=0 if w.order == 4 else 1024)
-        log(f"split: t={s['tsys']:.3e} after {s['num_bstep']} block steps: {s['error'] or 'ok'}")
+        # same span, same checks; a run that collapses or crawls (steps
+        # pinned at dt_min) is stopped at 3x the FP64 wall or 4x its block
+        # steps and reported as such
+        log(f"split-precision integration t=0->{tend:g}" + (" (event-timed)" if single else ""))
+        s = integrate(w, "split", states, tend, timing=single, graph=graph, chunks=chunks, t_energy=t_cfg,
+                      wall_limit=max(60.0, 3 * f["wall_s"]), max_bsteps=4 * f["num_bstep"] + 1024)
+        log(f"split: t={s['tsys']:.3e} after {s['num_bstep']} block steps, {s['wall_s']:.3f} s: "
+            f"{s['error'] or 'ok'}, dE/E {s.get('dE_E')}")
         split = {"integration_error": s["error"], "tsys_reached": s["tsys"], "block_steps": s["num_bstep"],
-                 "interactions_per_s": s["interactions"] / s["wall_s"], "dE_E": s.get("dE_E"),
-                 "dE_E_median": s.get("dE_E_median")}
+                 "wall_s": s["wall_s"], "interactions_per_s": s["interactions"] / s["wall_s"],
+                 "over_fp64": (s["interactions"] / s["wall_s"]) / value,
+                 "dE_E": s.get("dE_E"), "dE_E_median": s.get("dE_E_median"),
+                 f"dE_E_at_t{t_cfg:g}": s.get("dE_E_at_t_energy"),
+                 f"dE_E_at_t{t_cfg:g}_over_fp64": (None if s.get("dE_E_at_t_energy") is None
+                                                  else s["dE_E_at_t_energy"] / f["dE_E_at_t_energy"])}
+        if s["error"] is None:
+            st_ = s if single else integrate(w, "split", states, tend, timing=True)
+            s_ms, s_ach = kernel_rate(st_, w.order)
+            split_roof = split_roofline(w, s_ms, s_ach, st_["force_launches"])
         if len(states) == 1:
             ap2 = all_pairs(w.order, states[0][0], states[0][1])
             split.update({"all_pairs_interactions_per_s": ap2["split"]["interactions_per_s"],
@@ -462,6 +526,7 @@ def main():
                    "n": n_j, "clusters": len(w.clusters), "kernel": "fp64"},
         "gflops": value * H.FLOPS[w.order] * 1e-9,
         "dE_E": f["dE_E"],
+        f"dE_E_at_t{t_cfg:g}": f.get("dE_E_at_t_energy"),
         "block_steps": f["num_bstep"],
         "mean_nact": f["num_step"] / max(f["num_bstep"], 1),
         "us_per_block_step": 1e6 * f["wall_s"] / max(f["num_bstep"], 1),
@@ -473,6 +538,7 @@ def main():
                      "force_share_of_timed_wall": ft["force_ms"] * 1e-3 / ft["wall_s"],
                      "by_active_count": by_active_count(ft["log"], n_j)},
         "split": split,
+        "roofline_split": split_roof,
         "all_pairs_fp64": None if ap2 is None else {
             "interactions_per_s": ap2["fp64"]["interactions_per_s"], "gflops": ap2["fp64"]["gflops"],
             "force_ms": ap2["fp64"]["force_ms"], "frac": ap2["fp64"]["gflops"] * 1e-3 / PEAK_FP64_TFLOPS},

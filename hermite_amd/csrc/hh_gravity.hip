@@ -362,11 +362,10 @@ static int launch_potential(hh_ctx *c, double eps2)
     if (c->order == HH_ORDER_8DD)  // hi + lo positions, self excluded by index
         return c->variant == HH_EXACT ? launch_tile<1, EXACT, false>(c, g, c->d_jst, c->nbody, eps2, c->d_pot, nullptr)
                                       : launch_tile<1, FP64, false>(c, g, c->d_jst, c->nbody, eps2, c->d_pot, nullptr);
-    switch (c->variant) {
-    case HH_FP64: return launch_tile<0, FP64, false>(c, g, c->d_jst, c->nbody, eps2, c->d_pot, nullptr);
-    case HH_SPLIT: return launch_tile<0, SPLIT, false>(c, g, c->d_jst, c->nbody, eps2, c->d_pot, nullptr);
-    default: return launch_tile<0, EXACT, false>(c, g, c->d_jst, c->nbody, eps2, c->d_pot, nullptr);
-    }
+    // the potential is the energy diagnostic: FP64 for the split variant too,
+    // so that its energy error measures the integration, not the diagnostic
+    return c->variant == HH_EXACT ? launch_tile<0, EXACT, false>(c, g, c->d_jst, c->nbody, eps2, c->d_pot, nullptr)
+                                  : launch_tile<0, FP64, false>(c, g, c->d_jst, c->nbody, eps2, c->d_pot, nullptr);
 }
 
 static int check_ctx(hh_ctx *c)
@@ -1221,11 +1220,9 @@ int hhr_energy(hhr_ctx *c, int k, double *energy)
     tmp.part_cap = c->part_cap;
     int r;
     const Geo g{1, 1, 0};
-    switch (c->variant) {
-    case HH_FP64: r = launch_tile<0, FP64, false>(&tmp, g, c->rp + o, nk, c->eps2[k], c->pot, nullptr); break;
-    case HH_SPLIT: r = launch_tile<0, SPLIT, false>(&tmp, g, c->rp + o, nk, c->eps2[k], c->pot, nullptr); break;
-    default: r = launch_tile<0, EXACT, false>(&tmp, g, c->rp + o, nk, c->eps2[k], c->pot, nullptr); break;
-    }
+    // FP64 potential for the split variant too (the energy diagnostic)
+    if (c->variant == HH_EXACT) r = launch_tile<0, EXACT, false>(&tmp, g, c->rp + o, nk, c->eps2[k], c->pot, nullptr);
+    else r = launch_tile<0, FP64, false>(&tmp, g, c->rp + o, nk, c->eps2[k], c->pot, nullptr);
     tmp.stream = nullptr;
     tmp.d_part = nullptr;
     if (r != HH_OK) return rfail(c, r, tmp.err);

@@ -333,75 +333,118 @@ __device__ __forceinline__ void body_fp64(const double *p, const double *q, doub
     }
 }
 
-// SPLIT: relative position, velocity, acceleration and jerk formed in FP64
-// and rounded to FP32; r^-1 from v_rsq_f32; the rest of the chain rule in
-// FP32.  A holds FP32 partial sums, folded into FP64 by the caller.
+// SPLIT: per-family precision.  A relative error of 1e-7 in the
+// acceleration, the level of an FP32 interaction body, makes the 6th- and
+// 8th-order correctors collapse (their reconstructed derivatives scale the
+// force differences by (2/dt)^5 and (2/dt)^7, SRC/hermite6.h:168-170,
+// SRC/hermite8.h:203-206), while the jerk tolerates 1e-6 at 6th order and
+// snap and crackle 1e-5.  So the acceleration family (and at 8th order the
+// jerk) is evaluated and accumulated in FP64: FP64 r^2, r^-1 from the
+// v_rsq_f32 seed with one Newton step (relative error <= 1.5 h^2 ~ 2^-44,
+// h = 1 - r^2 y0^2), FP64 m r^-3.  The rest of the chain rule runs in FP32:
+// relative velocity, acceleration and jerk from the FP32-rounded
+// predictions, relative position and the scalars rounded from FP64.  Force
+// words [split_lo, F) go to the FP32 partial sums Af, which the caller folds
+// into FP64 every few j; everything else goes to the FP64 accumulators A.
+__host__ __device__ constexpr int split_lo(int o) { return o == 8 ? 6 : 3; }
+__host__ __device__ constexpr int n_f32(int o) { return (o == 4 || o == 6 || o == 8) ? f_words(o) - split_lo(o) : 1; }
+
 template <int ORDER, bool POT>
-__device__ __forceinline__ void body_split(const double *p, const double *q, float eps2, bool self, int mask_r2, float *A)
+__device__ __forceinline__ void body_split(const double *p, const double *q, double eps2, bool self, double *A, float *Af)
 {
-    static_assert(ORDER != 1 && ORDER != 9, "no split-precision body for hi + lo positions");
-    const float dx = (float)(q[0] - p[0]), dy = (float)(q[1] - p[1]), dz = (float)(q[2] - p[2]);
-    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, eps2)));
-    const float rinv = __builtin_amdgcn_rsqf(r2);
-    const float mj = (float)q[3];
-    if constexpr (ORDER == 0) {
-        const bool keep = mask_r2 ? (r2 > eps2) : !self;
-        A[0] = __builtin_fmaf(keep ? -mj : 0.0f, rinv, A[0]);
-        return;
+    static_assert(ORDER == 4 || ORDER == 6 || ORDER == 8, "split-precision body for the force orders only");
+    const double dx = q[0] - p[0], dy = q[1] - p[1], dz = q[2] - p[2];
+    const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, __builtin_fma(dx, dx, eps2)));
+    const float yf = __builtin_amdgcn_rsqf((float)r2);
+    const double y = (double)yf;
+    const double h = __builtin_fma(-r2, y * y, 1.0);  // y*y is exact: y has 24 significant bits
+    const double rinv = __builtin_fma(y * h, 0.5, y);
+    const double rinv2 = rinv * rinv;
+    const double mr = q[3] * rinv;
+    const double mr3 = mr * rinv2;
+    if constexpr (POT) A[f_words(ORDER)] = __builtin_fma(self ? 0.0 : -1.0, mr, A[f_words(ORDER)]);
+    A[0] = __builtin_fma(mr3, dx, A[0]);
+    A[1] = __builtin_fma(mr3, dy, A[1]);
+    A[2] = __builtin_fma(mr3, dz, A[2]);
+    const float dxf = (float)dx, dyf = (float)dy, dzf = (float)dz;
+    const float ri2f = (float)rinv2, mr3f = (float)mr3;
+    if constexpr (ORDER == 4) {
+        const float dvx = (float)q[4] - (float)p[4], dvy = (float)q[5] - (float)p[5], dvz = (float)q[6] - (float)p[6];
+        const float drdv = __builtin_fmaf(dzf, dvz, __builtin_fmaf(dyf, dvy, dxf * dvx));
+        const float a3 = -3.0f * (drdv * ri2f);
+        Af[0] = __builtin_fmaf(mr3f, __builtin_fmaf(a3, dxf, dvx), Af[0]);
+        Af[1] = __builtin_fmaf(mr3f, __builtin_fmaf(a3, dyf, dvy), Af[1]);
+        Af[2] = __builtin_fmaf(mr3f, __builtin_fmaf(a3, dzf, dvz), Af[2]);
+    } else if constexpr (ORDER == 6) {
+        const float dvx = (float)q[4] - (float)p[4], dvy = (float)q[5] - (float)p[5], dvz = (float)q[6] - (float)p[6];
+        const float dax = (float)q[7] - (float)p[7], day = (float)q[8] - (float)p[8], daz = (float)q[9] - (float)p[9];
+        const float drdv = __builtin_fmaf(dzf, dvz, __builtin_fmaf(dyf, dvy, dxf * dvx));
+        const float alpha = drdv * ri2f;
+        const float s = __builtin_fmaf(dzf, daz, __builtin_fmaf(dyf, day, __builtin_fmaf(dxf, dax,
+                        __builtin_fmaf(dvz, dvz, __builtin_fmaf(dvy, dvy, dvx * dvx)))));
+        const float beta = __builtin_fmaf(s, ri2f, alpha * alpha);
+        const float a3 = -3.0f * alpha;
+        const float tx = __builtin_fmaf(a3, dxf, dvx), ty = __builtin_fmaf(a3, dyf, dvy), tz = __builtin_fmaf(a3, dzf, dvz);
+        Af[0] = __builtin_fmaf(mr3f, tx, Af[0]);
+        Af[1] = __builtin_fmaf(mr3f, ty, Af[1]);
+        Af[2] = __builtin_fmaf(mr3f, tz, Af[2]);
+        const float a6 = 2.0f * a3, b3 = -3.0f * beta;
+        Af[3] = __builtin_fmaf(mr3f, __builtin_fmaf(b3, dxf, __builtin_fmaf(a6, tx, dax)), Af[3]);
+        Af[4] = __builtin_fmaf(mr3f, __builtin_fmaf(b3, dyf, __builtin_fmaf(a6, ty, day)), Af[4]);
+        Af[5] = __builtin_fmaf(mr3f, __builtin_fmaf(b3, dzf, __builtin_fmaf(a6, tz, daz)), Af[5]);
     } else {
-        const float dvx = (float)(q[4] - p[4]), dvy = (float)(q[5] - p[5]), dvz = (float)(q[6] - p[6]);
-        const float rinv2 = rinv * rinv;
-        const float mr = mj * rinv;
-        const float mr3 = mr * rinv2;
-        if constexpr (POT) A[f_words(ORDER)] = __builtin_fmaf(self ? 0.0f : -1.0f, mr, A[f_words(ORDER)]);
-        const float drdv = __builtin_fmaf(dz, dvz, __builtin_fmaf(dy, dvy, dx * dvx));
-        A[0] = __builtin_fmaf(mr3, dx, A[0]);
-        A[1] = __builtin_fmaf(mr3, dy, A[1]);
-        A[2] = __builtin_fmaf(mr3, dz, A[2]);
-        if constexpr (ORDER == 4) {
-            const float a3 = -3.0f * (drdv * rinv2);
-            A[3] = __builtin_fmaf(mr3, __builtin_fmaf(a3, dx, dvx), A[3]);
-            A[4] = __builtin_fmaf(mr3, __builtin_fmaf(a3, dy, dvy), A[4]);
-            A[5] = __builtin_fmaf(mr3, __builtin_fmaf(a3, dz, dvz), A[5]);
-        } else {
-            const float dax = (float)(q[7] - p[7]), day = (float)(q[8] - p[8]), daz = (float)(q[9] - p[9]);
-            const float alpha = drdv * rinv2;
-            const float s = __builtin_fmaf(dz, daz, __builtin_fmaf(dy, day, __builtin_fmaf(dx, dax,
-                            __builtin_fmaf(dvz, dvz, __builtin_fmaf(dvy, dvy, dvx * dvx)))));
-            const float beta = __builtin_fmaf(s, rinv2, alpha * alpha);
-            const float a3 = -3.0f * alpha;
-            const float tx = __builtin_fmaf(a3, dx, dvx), ty = __builtin_fmaf(a3, dy, dvy), tz = __builtin_fmaf(a3, dz, dvz);
-            A[3] = __builtin_fmaf(mr3, tx, A[3]);
-            A[4] = __builtin_fmaf(mr3, ty, A[4]);
-            A[5] = __builtin_fmaf(mr3, tz, A[5]);
-            const float a6 = 2.0f * a3, b3 = -3.0f * beta;
-            const float ux = __builtin_fmaf(b3, dx, __builtin_fmaf(a6, tx, dax));
-            const float uy = __builtin_fmaf(b3, dy, __builtin_fmaf(a6, ty, day));
-            const float uz = __builtin_fmaf(b3, dz, __builtin_fmaf(a6, tz, daz));
-            A[6] = __builtin_fmaf(mr3, ux, A[6]);
-            A[7] = __builtin_fmaf(mr3, uy, A[7]);
-            A[8] = __builtin_fmaf(mr3, uz, A[8]);
-            if constexpr (ORDER == 8) {
-                const float djx = (float)(q[10] - p[10]), djy = (float)(q[11] - p[11]), djz = (float)(q[12] - p[12]);
-                const float dvda = __builtin_fmaf(dvz, daz, __builtin_fmaf(dvy, day, dvx * dax));
-                const float drdj = __builtin_fmaf(dz, djz, __builtin_fmaf(dy, djy, dx * djx));
-                const float g0 = __builtin_fmaf(3.0f, dvda, drdj);
-                const float g1 = __builtin_fmaf(-4.0f * alpha, alpha, 3.0f * beta);
-                const float gamma = __builtin_fmaf(g0, rinv2, alpha * g1);
-                const float a9 = 3.0f * a3, b9 = 3.0f * b3, g3 = -3.0f * gamma;
-                const float cx = __builtin_fmaf(g3, dx, __builtin_fmaf(b9, tx, __builtin_fmaf(a9, ux, djx)));
-                const float cy = __builtin_fmaf(g3, dy, __builtin_fmaf(b9, ty, __builtin_fmaf(a9, uy, djy)));
-                const float cz = __builtin_fmaf(g3, dz, __builtin_fmaf(b9, tz, __builtin_fmaf(a9, uz, djz)));
-                A[9] = __builtin_fmaf(mr3, cx, A[9]);
-                A[10] = __builtin_fmaf(mr3, cy, A[10]);
-                A[11] = __builtin_fmaf(mr3, cz, A[11]);
-            }
-        }
+        // the jerk in FP64 as well
+        const double dvx = q[4] - p[4], dvy = q[5] - p[5], dvz = q[6] - p[6];
+        const double drdv = __builtin_fma(dz, dvz, __builtin_fma(dy, dvy, dx * dvx));
+        const double alpha = drdv * rinv2;
+        const double a3 = -3.0 * alpha;
+        const double tx = __builtin_fma(a3, dx, dvx), ty = __builtin_fma(a3, dy, dvy), tz = __builtin_fma(a3, dz, dvz);
+        A[3] = __builtin_fma(mr3, tx, A[3]);
+        A[4] = __builtin_fma(mr3, ty, A[4]);
+        A[5] = __builtin_fma(mr3, tz, A[5]);
+        const float dvxf = (float)dvx, dvyf = (float)dvy, dvzf = (float)dvz;
+        const float txf = (float)tx, tyf = (float)ty, tzf = (float)tz, alphaf = (float)alpha;
+        const float dax = (float)q[7] - (float)p[7], day = (float)q[8] - (float)p[8], daz = (float)q[9] - (float)p[9];
+        const float djx = (float)q[10] - (float)p[10], djy = (float)q[11] - (float)p[11], djz = (float)q[12] - (float)p[12];
+        const float s = __builtin_fmaf(dzf, daz, __builtin_fmaf(dyf, day, __builtin_fmaf(dxf, dax,
+                        __builtin_fmaf(dvzf, dvzf, __builtin_fmaf(dvyf, dvyf, dvxf * dvxf)))));
+        const float beta = __builtin_fmaf(s, ri2f, alphaf * alphaf);
+        const float a6 = -6.0f * alphaf, b3 = -3.0f * beta;
+        const float ux = __builtin_fmaf(b3, dxf, __builtin_fmaf(a6, txf, dax));
+        const float uy = __builtin_fmaf(b3, dyf, __builtin_fmaf(a6, tyf, day));
+        const float uz = __builtin_fmaf(b3, dzf, __builtin_fmaf(a6, tzf, daz));
+        Af[0] = __builtin_fmaf(mr3f, ux, Af[0]);
+        Af[1] = __builtin_fmaf(mr3f, uy, Af[1]);
+        Af[2] = __builtin_fmaf(mr3f, uz, Af[2]);
+        const float dvda = __builtin_fmaf(dvzf, daz, __builtin_fmaf(dvyf, day, dvxf * dax));
+        const float drdj = __builtin_fmaf(dzf, djz, __builtin_fmaf(dyf, djy, dxf * djx));
+        // gamma = (3 dv.da + dr.dj)/r^2 + alpha (3 beta - 4 alpha^2)
+        const float g0 = __builtin_fmaf(3.0f, dvda, drdj);
+        const float g1 = __builtin_fmaf(-4.0f * alphaf, alphaf, 3.0f * beta);
+        const float gamma = __builtin_fmaf(g0, ri2f, alphaf * g1);
+        const float a9 = -9.0f * alphaf, b9 = 3.0f * b3, g3 = -3.0f * gamma;
+        Af[3] = __builtin_fmaf(mr3f, __builtin_fmaf(g3, dxf, __builtin_fmaf(b9, txf, __builtin_fmaf(a9, ux, djx))), Af[3]);
+        Af[4] = __builtin_fmaf(mr3f, __builtin_fmaf(g3, dyf, __builtin_fmaf(b9, tyf, __builtin_fmaf(a9, uy, djy))), Af[4]);
+        Af[5] = __builtin_fmaf(mr3f, __builtin_fmaf(g3, dzf, __builtin_fmaf(b9, tzf, __builtin_fmaf(a9, uz, djz))), Af[5]);
+    }
+}
+
+// Adds the FP32 partial sums of the split body to the FP64 accumulators and
+// clears them.
+template <int ORDER>
+__device__ __forceinline__ void split_fold(double *A, float *Af)
+{
+#pragma unroll
+    for (int c = 0; c < n_f32(ORDER); c++) {
+        A[split_lo(ORDER) + c] += (double)Af[c];
+        Af[c] = 0.0f;
     }
 }
 
 // SPLIT: j per FP32 partial sum before it is folded into the FP64
-// accumulators (a template parameter FOLD of tile_kernel, tuned per order).
+// accumulators (a template parameter FOLD of tile_kernel, tuned per order;
+// JFOLD consecutive j of a lane in the j-parallel kernels).
+constexpr int JFOLD = 8;
 
 // ---- i-parallel tile kernel ----------------------------------------------
 // Workgroup (iblock, slice): NT threads each own NI i-particles
@@ -449,7 +492,6 @@ __global__ __launch_bounds__(NT) void tile_kernel(TileArgs a)
 #pragma unroll
         for (int w = 0; w < Q; w++) jr[w] = a.src[(size_t)src_word(ORDER, w) * npad + t0 * TJ + tid];
     }
-    const float eps2f = (float)a.eps2;
     // a wave whose lanes are all past the i range only helps stage tiles
     const bool busy = iblock * NT * NI + (tid & ~63) < a.nact;
 
@@ -485,12 +527,13 @@ __global__ __launch_bounds__(NT) void tile_kernel(TileArgs a)
                     body_fp64<ORDER, POT>(pi[k], q, a.eps2, jbase + jj == ii[k], a.pot_mask_r2, A[k]);
             }
         } else {
+            static_assert(ORDER > 1, "the potential runs in FP64 for every variant");
             for (int jb = 0; jb < TJ; jb += FOLD) {
-                float Af[NI][NC];
+                float Af[NI][n_f32(ORDER)];
 #pragma unroll
                 for (int k = 0; k < NI; k++)
 #pragma unroll
-                    for (int c = 0; c < NC; c++) Af[k][c] = 0.0f;
+                    for (int c = 0; c < n_f32(ORDER); c++) Af[k][c] = 0.0f;
 #pragma unroll 2
                 for (int jj = jb; jj < jb + FOLD; jj++) {
                     double q[L];
@@ -498,12 +541,10 @@ __global__ __launch_bounds__(NT) void tile_kernel(TileArgs a)
                     for (int w = 0; w < L; w++) q[w] = sj[jj * L + w];
 #pragma unroll
                     for (int k = 0; k < NI; k++)
-                        body_split<ORDER, POT>(pi[k], q, eps2f, jbase + jj == ii[k], a.pot_mask_r2, Af[k]);
+                        body_split<ORDER, POT>(pi[k], q, a.eps2, jbase + jj == ii[k], A[k], Af[k]);
                 }
 #pragma unroll
-                for (int k = 0; k < NI; k++)
-#pragma unroll
-                    for (int c = 0; c < NC; c++) A[k][c] += (double)Af[k][c];
+                for (int k = 0; k < NI; k++) split_fold<ORDER>(A[k], Af[k]);
             }
         }
     }
@@ -627,24 +668,32 @@ __global__ __launch_bounds__(NT) void jpar_kernel(JParArgs a)
 
     const int j0 = slice * a.jchunk;
     const int j1 = min(j0 + a.jchunk, npad);
-    const float eps2f = (float)a.eps2;
+    float Af[GI][n_f32(ORDER)];
+#pragma unroll
+    for (int g = 0; g < GI; g++)
+#pragma unroll
+        for (int c = 0; c < n_f32(ORDER); c++) Af[g][c] = 0.0f;
+    int nfold = 0;
     for (int j = j0 + tid; j < j1; j += NT) {
         double q[Q];
 #pragma unroll
         for (int w = 0; w < Q; w++) q[w] = a.src[(size_t)w * npad + j];
 #pragma unroll
         for (int g = 0; g < GI; g++) {
-            if constexpr (MODE == FP64) {
-                body_fp64<ORDER, POT>(pi[g], q, a.eps2, j == ii[g], 0, A[g]);
-            } else {
-                float Af[NC];
+            if constexpr (MODE == FP64) body_fp64<ORDER, POT>(pi[g], q, a.eps2, j == ii[g], 0, A[g]);
+            else body_split<ORDER, POT>(pi[g], q, a.eps2, j == ii[g], A[g], Af[g]);
+        }
+        if constexpr (MODE == SPLIT) {
+            if (++nfold == JFOLD) {
+                nfold = 0;
 #pragma unroll
-                for (int c = 0; c < NC; c++) Af[c] = 0.0f;
-                body_split<ORDER, POT>(pi[g], q, eps2f, j == ii[g], 0, Af);
-#pragma unroll
-                for (int c = 0; c < NC; c++) A[g][c] += (double)Af[c];
+                for (int g = 0; g < GI; g++) split_fold<ORDER>(A[g], Af[g]);
             }
         }
+    }
+    if constexpr (MODE == SPLIT) {
+#pragma unroll
+        for (int g = 0; g < GI; g++) split_fold<ORDER>(A[g], Af[g]);
     }
 
     // workgroup sum in a fixed order: shuffle tree per wave, then waves 0..3

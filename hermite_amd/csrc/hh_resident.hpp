@@ -645,7 +645,6 @@ __device__ __forceinline__ void r_force_body(const RArgs &a)
         const int local = task - c.task0;
         const int grp = local / c.nslice, slice = local - grp * c.nslice;
         const double eps2 = c.eps2;
-        const float eps2f = (float)eps2;
         const double *src = a.jpr;
 
         if (c.mode == 1) {
@@ -708,23 +707,21 @@ __device__ __forceinline__ void r_force_body(const RArgs &a)
                     }
                 } else {
                     for (int jb = jlo; jb < jhi; jb += FOLD) {
-                        float Af[NI][NC];
+                        float Af[NI][n_f32(ORDER)];
 #pragma unroll
                         for (int k = 0; k < NI; k++)
 #pragma unroll
-                            for (int q = 0; q < NC; q++) Af[k][q] = 0.0f;
+                            for (int q = 0; q < n_f32(ORDER); q++) Af[k][q] = 0.0f;
 #pragma unroll 2
                         for (int jj = jb; jj < jb + FOLD; jj++) {
                             double q[L];
 #pragma unroll
                             for (int w = 0; w < L; w++) q[w] = sj[jj * L + w];
 #pragma unroll
-                            for (int k = 0; k < NI; k++) body_split<ORDER, false>(pi[k], q, eps2f, false, 0, Af[k]);
+                            for (int k = 0; k < NI; k++) body_split<ORDER, false>(pi[k], q, eps2, false, A[k], Af[k]);
                         }
 #pragma unroll
-                        for (int k = 0; k < NI; k++)
-#pragma unroll
-                            for (int q = 0; q < NC; q++) A[k][q] += (double)Af[k][q];
+                        for (int k = 0; k < NI; k++) split_fold<ORDER>(A[k], Af[k]);
                     }
                 }
             }
@@ -774,23 +771,32 @@ __device__ __forceinline__ void r_force_body(const RArgs &a)
                 for (int q = 0; q < NC; q++) A[g][q] = 0.0;
             const int j0 = c.off + slice * c.jchunk;
             const int j1 = min(j0 + c.jchunk, c.off + c.ntiles * TJ);
+            float Af[R_GI][n_f32(ORDER)];
+#pragma unroll
+            for (int g = 0; g < R_GI; g++)
+#pragma unroll
+                for (int qq = 0; qq < n_f32(ORDER); qq++) Af[g][qq] = 0.0f;
+            int nfold = 0;
             for (int j = j0 + tid; j < j1; j += NT) {
                 double q[Q];
 #pragma unroll
                 for (int w = 0; w < Q; w++) q[w] = src[(size_t)w * npad + j];
 #pragma unroll
                 for (int g = 0; g < R_GI; g++) {
-                    if constexpr (MODE == FP64) {
-                        body_fp64<ORDER, false>(pi[g], q, eps2, false, 0, A[g]);
-                    } else {
-                        float Af[NC];
+                    if constexpr (MODE == FP64) body_fp64<ORDER, false>(pi[g], q, eps2, false, 0, A[g]);
+                    else body_split<ORDER, false>(pi[g], q, eps2, false, A[g], Af[g]);
+                }
+                if constexpr (MODE == SPLIT) {
+                    if (++nfold == JFOLD) {
+                        nfold = 0;
 #pragma unroll
-                        for (int qq = 0; qq < NC; qq++) Af[qq] = 0.0f;
-                        body_split<ORDER, false>(pi[g], q, eps2f, false, 0, Af);
-#pragma unroll
-                        for (int qq = 0; qq < NC; qq++) A[g][qq] += (double)Af[qq];
+                        for (int g = 0; g < R_GI; g++) split_fold<ORDER>(A[g], Af[g]);
                     }
                 }
+            }
+            if constexpr (MODE == SPLIT) {
+#pragma unroll
+                for (int g = 0; g < R_GI; g++) split_fold<ORDER>(A[g], Af[g]);
             }
             __syncthreads();  // red reuse across tasks
 #pragma unroll

@@ -6,9 +6,12 @@ Tolerances (stated per variant):
   FP64   per particle, |F_gpu - F_oracle| / |F_oracle| per vector family:
          acc, jrk <= 1e-12; snp, crk <= 1e-10 (these two suffer cancellation,
          sum |terms| >> |sum|, so they are also checked against sum|terms|).
-  SPLIT  per-particle relative force (acc) error <= 1e-6 (the target bound);
-         jrk <= 5e-6 (FP32 rounding of the pair terms is amplified by the
-         cancellation inside dv - 3 (dr.dv/r^2) dr); snp/crk median <= 1e-5.
+  SPLIT  the acceleration (and at 8th order the jerk) is FP64 arithmetic:
+         acc (8th: and jrk) <= 1e-11 per particle, far inside the 1e-6
+         target bound.  The families evaluated in FP32: jrk (4th, 6th)
+         <= 1e-5 per particle (FP32 rounding of the pair terms is amplified
+         by the cancellation inside dv - 3 (dr.dv/r^2) dr); snp/crk median
+         <= 1e-5.  Potential (fused or stand-alone): FP64, 1e-12.
 """
 import numpy as np
 import pytest
@@ -93,21 +96,37 @@ def test_split_tolerance(order):
     errs = fam_err(F, R, order)
     print(f"split order={order} max fam err", [f"{e.max():.2e}" for e in errs],
           "median", [f"{np.median(e):.2e}" for e in errs], flush=True)
-    assert errs[0].max() <= 1e-6 and errs[1].max() <= 5e-6
-    for e in errs[2:]:
-        assert np.median(e) <= 1e-5
+    check_split(errs, order)
+
+
+def check_split(errs, order, fp32_max=1e-5):
+    """Per-family bounds of the split variant (module docstring)."""
+    n64 = 2 if order == 8 else 1  # families evaluated in FP64
+    for e in errs[:n64]:
+        assert e.max() <= 1e-11, e.max()
+    if order < 8:
+        assert errs[1].max() <= fp32_max, errs[1].max()
+    for e in errs[max(n64, 2):]:
+        assert np.median(e) <= 1e-5, np.median(e)
 
 
 @pytest.mark.parametrize("order", ORDERS)
 def test_predictor_exact_and_fp64(order):
-    """Predicted state feeds the force; EXACT must reproduce the oracle's
-    predictor bit for bit (checked through the force on a 2-body system,
-    where every predicted word enters)."""
+    """Predicted state feeds the force, in which every predicted word enters.
+    EXACT must reproduce the oracle's predictor bit for bit; the FP64
+    predictor (FMA Horner) must agree with it to the FP64 force tolerance
+    (acc, jrk <= 1e-12, snp, crk <= 1e-10 per particle)."""
     eps2 = 1e-4
     P = hermite_state(order, 300, seed=9, spread=0.25)
     for tsys in (0.0, 1 / 32):
+        R = O.force(order, O.predict(order, P, tsys), 300, eps2)
         F = gpu_forces(order, P, "exact", tsys, 300, eps2)
-        assert np.array_equal(F, O.force(order, O.predict(order, P, tsys), 300, eps2))
+        assert np.array_equal(F, R)
+        errs = fam_err(gpu_forces(order, P, "fp64", tsys, 300, eps2), R, order)
+        print(f"fp64 predictor order={order} tsys={tsys}", [f"{e.max():.2e}" for e in errs], flush=True)
+        assert errs[0].max() <= 1e-12 and errs[1].max() <= 1e-12
+        for e in errs[2:]:
+            assert e.max() <= 1e-10
 
 
 @pytest.mark.parametrize("variant", ["fp64", "split", "exact"])
@@ -124,11 +143,11 @@ def test_fused_potential(variant):
     if variant == "exact":
         assert np.array_equal(pot, ref)
     else:
-        tol = {"fp64": 1e-13, "split": 1e-6}[variant]
+        tol = {"fp64": 1e-13, "split": 1e-12}[variant]
         assert np.max(np.abs(pot - ref) / np.abs(ref)) <= tol
 
 
-@pytest.mark.parametrize("variant,tol", [("fp64", 1e-13), ("split", 1e-6)])
+@pytest.mark.parametrize("variant,tol", [("fp64", 1e-13), ("split", 1e-13)])
 def test_potential_variants(variant, tol):
     for order in ORDERS:
         eps2 = (1 / 64) ** 2
@@ -179,13 +198,20 @@ def test_repeatable_and_dirty_upload():
 
 def test_full_size_properties():
     """6th order, N = 65536 Plummer (seed 2, eps = 1/1024): the workload the
-    benchmark runs.  Checks that cost nothing on the CPU at this size:
-    momentum conservation of every force family, agreement with the oracle on
-    a sample of i-particles, and FP64/SPLIT agreement."""
+    benchmark runs, in the particle order it runs in (after init_step, which
+    groups the particles by dt level, SRC/nbodysystem.h:565-631; the j
+    summation order follows it).  Checks that cost nothing on the CPU at this
+    size: momentum conservation of every force family, agreement with the
+    oracle on a sample of i-particles, and FP64/SPLIT agreement."""
     order, n = 6, 65536
     eps2 = (1 / 1024) ** 2
     m, x, v = H.plummer(n, 2)
-    P = O.make_particles(order, m, x, v)
+    s = H.System(order, O.make_particles(order, m, x, v), 0.0, eps2, 0.2, 0.1, 1 / 16, variant="fp64")
+    s.init_step()
+    P = s.ptcl()
+    s.close()
+    m = P[:, O.W_MASS]
+    assert not np.array_equal(np.sort(O.ids_of(P)), O.ids_of(P)), "init_step left the generator order"
     g = H.Gravity(order, n, "fp64")
     g.set_jp_all(P)
     g.predict_all(0.0)
@@ -208,11 +234,12 @@ def test_full_size_properties():
     es = fam_err(S, F, order)
     print("split vs fp64 at N=65536: max", [f"{e.max():.2e}" for e in es], "median",
           [f"{np.median(e):.2e}" for e in es], flush=True)
-    assert es[0].max() <= 1e-6
+    assert es[0].max() <= 1e-6  # the target bound (FP64 arithmetic: ~1e-13)
+    check_split(es, order, fp32_max=1e-4)
 
 
 @pytest.mark.parametrize("order", ORDERS)
-@pytest.mark.parametrize("variant,tol", [("fp64", 1e-12), ("split", 5e-6)])
+@pytest.mark.parametrize("variant,tol", [("fp64", 1e-12), ("split", 1e-11)])
 def test_both_kernels_all_small_nact(order, variant, tol):
     """The i-parallel (1) and j-parallel (2) kernels, forced, on ragged
     active counts including the fused potential."""
@@ -233,7 +260,9 @@ def test_both_kernels_all_small_nact(order, variant, tol):
             F, pot = g.calc_force_on_first_nact(nact, eps2, with_pot=True)
             assert g.last_geometry()["kernel"] == kernel
             errs = fam_err(F, R[:nact], order)
-            assert errs[0].max() <= tol and errs[1].max() <= tol, (kernel, nact, errs[0].max(), errs[1].max())
+            # split: jrk is FP32 arithmetic below 8th order
+            tol1 = 1e-5 if variant == "split" and order < 8 else tol
+            assert errs[0].max() <= tol and errs[1].max() <= tol1, (kernel, nact, errs[0].max(), errs[1].max())
             assert np.max(np.abs(pot - potref[:nact]) / np.abs(potref[:nact])) <= tol
     g.close()
 

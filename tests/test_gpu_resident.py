@@ -82,33 +82,31 @@ def test_resident_fp64_statistics(order):
     r.close()
 
 
-def test_resident_split_behaviour():
-    """The FP32 interaction body leaves ~1e-7 relative noise in each force
-    (the split parity tests bound it).  The 4th-order scheme integrates with
-    it, at a larger energy error; the 8th-order derivative reconstruction
-    (d4a..d7a ~ force differences / h^4..h^7) amplifies it until the state
-    runs away, which the driver reports instead of integrating garbage."""
+@pytest.mark.parametrize("order", ORDERS)
+def test_resident_split_integrates(order):
+    """The split kernel keeps the acceleration (8th order: and the jerk) in
+    FP64 and the higher families in FP32, within the noise each family
+    tolerates (tests/test_noise_sensitivity.py), so every order integrates
+    the reference's pl1k run (t = 0 -> 1): the same mean active count as the
+    reference log to 1 %, and |dE/E| within twice the FP64 run's."""
+    eta, eta_s, want1, _ = NACT_KAT[order]
     n = 1024
-    m, x, v = H.plummer(n, 19640916, 100.0)
     eps2 = (4.0 / n) ** 2
-    res = {}
-    for order in (4, 6):
-        eta, eta_s, want1, _ = NACT_KAT[order]
-        o = initialised(order, n, 19640916, eps2, eta, eta_s, 1 / 16, rmax=100.0)
-        r = H.Resident(order, "split", [(o.ptcl(), eps2, eta, 1 / 16, 0.0, 0.0)])
-        r.run(1.0)
-        st = r.stats()
-        res[order] = abs((r.energy() - o.init_energy) / o.init_energy)
-        print(f"split order={order}: nact={st['num_step'] / st['num_bstep']:.4e} |dE/E|={res[order]:.3e}", flush=True)
+    o = initialised(order, n, 19640916, eps2, eta, eta_s, 1 / 16, rmax=100.0)
+    P0, e0 = o.ptcl(), o.init_energy
+    de = {}
+    for variant in ("fp64", "split"):
+        r = H.Resident(order, variant, [(P0, eps2, eta, 1 / 16, 0.0, 0.0)])
+        try:
+            r.run(1.0)
+            st = r.stats()
+            de[variant] = abs((r.energy() - e0) / e0)
+        finally:
+            r.close()
+        print(f"{variant} order={order}: nact={st['num_step'] / st['num_bstep']:.4e} (reference {want1}) "
+              f"|dE/E|={de[variant]:.3e}", flush=True)
         assert abs(st["num_step"] / st["num_bstep"] / float(want1) - 1) < 0.01
-        assert res[order] < 1e-6
-        r.close()
-    eta, eta_s, _, _ = NACT_KAT[8]
-    o = initialised(8, n, 19640916, eps2, eta, eta_s, 1 / 16, rmax=100.0)
-    r = H.Resident(8, "split", [(o.ptcl(), eps2, eta, 1 / 16, 0.0, 0.0)])
-    with pytest.raises(H.HHError, match="non-finite|stopped advancing"):
-        r.run(1.0)
-    r.close()
+    assert de["split"] <= 2 * de["fp64"] + 1e-12, de
 
 
 def test_ensemble_equals_clusters_run_alone():
