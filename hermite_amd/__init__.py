@@ -60,6 +60,8 @@ def hip_lib():
         L.hh_mark_dirty.argtypes = [vp, C.c_int, C.c_int]
         L.hh_predict_all.argtypes = [vp, C.c_double]
         L.hh_force_first_nact.argtypes = [vp, C.c_int, C.c_double, _dp, _dp]
+        L.hh_force_first_nact_static.argtypes = [vp, C.c_int, C.c_double, _dp, _dp]
+        L.hh_static_scratch_doubles.restype = C.c_longlong
         L.hh_potential.argtypes = [vp, C.c_double, _dp]
         L.hh_last_error.argtypes = [vp]
         L.hh_last_error.restype = C.c_char_p

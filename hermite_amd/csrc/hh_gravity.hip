@@ -44,6 +44,11 @@ struct hh_ctx {
     double *d_pot = nullptr;    // npad
     double *d_part = nullptr;   // partial sums of j-sliced launches
     size_t part_cap = 0;        // doubles
+    // hh_force_first_nact_static: while set, j-sliced launches take their
+    // partial sums from the device's static scratch (static_cap doubles),
+    // with the j-slice count capped to fit, instead of growing d_part
+    double *static_part = nullptr;
+    size_t static_cap = 0;
     std::atomic<int> dirty_hi{0};
     int tune_kernel = 0, tune_split = 0, tune_ni = 0, tune_fold = 0;
     std::vector<TuneEntry> table;  // ascending nact_hi; empty = built-in choice
@@ -67,6 +72,17 @@ __global__ void rsqrt_eval_kernel(const double *__restrict__ x, double *__restri
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) y[i] = hh::rsqrt64(x[i]);
 }
+
+// Scratch of hh_force_first_nact_static: the j-slice partial sums of one
+// force call, in a static device array instead of a buffer that grows with
+// hipMalloc.  Every GPU the library runs on holds its own copy until the
+// process ends; host code reaches it only through the address that
+// hipGetSymbolAddress returns for the current device (the array's name in
+// host code is a stand-in, not a device address).
+constexpr size_t HH_SCRATCH_DOUBLES = (size_t)4 << 20;  // 32 MiB
+static __device__ __align__(256) double scratch[HH_SCRATCH_DOUBLES];
+constexpr int HH_MAX_DEVICES = 64;
+static std::atomic<double *> g_scratch_ptr[HH_MAX_DEVICES];
 
 #define HIPCHK(c, call)                                                                     \
     do {                                                                                    \
@@ -167,6 +183,32 @@ static int ensure_part(hh_ctx *c, size_t need)
     return HH_OK;
 }
 
+// Partial sums for a launch of nslice j-slices of `per_slice` doubles each:
+// the context's growing buffer, or the static scratch during
+// hh_force_first_nact_static, where the slice count is first capped to
+// what the scratch holds (cap_slices).
+static int cap_slices(const hh_ctx *c, int nslice, size_t per_slice)
+{
+    if (!c->static_part || nslice <= 1) return nslice;
+    const size_t most = c->static_cap / per_slice;
+    return (int)std::max<size_t>(1, std::min<size_t>((size_t)nslice, most));
+}
+
+static int get_part(hh_ctx *c, size_t need, double *__restrict *part)
+{
+    if (c->static_part) {
+        if (need > c->static_cap)
+            return fail(c, HH_E_LIMIT, "partial sums of " + std::to_string(need) + " doubles exceed the static scratch of " +
+                                          std::to_string(c->static_cap));
+        *part = c->static_part;
+        return HH_OK;
+    }
+    const int r = ensure_part(c, need);
+    if (r) return r;
+    *part = c->d_part;
+    return HH_OK;
+}
+
 // Workgroups wanted per force launch: several per CU on 256 CUs.
 static constexpr int TARGET_WG = 2048;
 
@@ -192,6 +234,7 @@ static int launch_tile_ni(hh_ctx *c, const Geo &g, const double *src, int ni, do
     if (MODE == EXACT) nslice = 1;
     else if (g.jsplit > 0) nslice = std::min(g.jsplit, ntiles);
     else nslice = std::max(1, std::min(ntiles, (TARGET_WG + iblocks - 1) / iblocks));
+    nslice = cap_slices(c, nslice, (size_t)NC * iblocks * bi);
     const int tps = (ntiles + nslice - 1) / nslice;
     nslice = (ntiles + tps - 1) / tps;
 
@@ -210,9 +253,8 @@ static int launch_tile_ni(hh_ctx *c, const Geo &g, const double *src, int ni, do
     a.fout = fout;
     a.pout = pout;
     if (nslice > 1) {
-        const int r = ensure_part(c, (size_t)nslice * NC * a.ipad);
+        const int r = get_part(c, (size_t)nslice * NC * a.ipad, &a.part);
         if (r) return r;
-        a.part = c->d_part;
     }
     hipLaunchKernelGGL((tile_kernel<ORDER, MODE, NI, POT, FOLD>), dim3(iblocks, nslice), dim3(NT), 0, c->stream, a);
     hipError_t e = hipGetLastError();
@@ -267,6 +309,7 @@ static int launch_jpar(hh_ctx *c, const Geo &g, int nact, double eps2)
     const int ntj = c->npad / NT;
     int nslice = g.jsplit > 0 ? g.jsplit : (TARGET_WG_JPAR + ngroups - 1) / ngroups;
     nslice = std::max(1, std::min(nslice, ntj));
+    nslice = cap_slices(c, nslice, (size_t)V * ngroups);
     const int jchunk = (ntj + nslice - 1) / nslice * NT;
     nslice = (c->npad + jchunk - 1) / jchunk;
     JParArgs a;
@@ -281,9 +324,8 @@ static int launch_jpar(hh_ctx *c, const Geo &g, int nact, double eps2)
     a.fout = c->d_force;
     a.pout = c->d_pot;
     if (nslice > 1) {
-        const int r = ensure_part(c, (size_t)nslice * V * ngroups);
+        const int r = get_part(c, (size_t)nslice * V * ngroups, &a.part);
         if (r) return r;
-        a.part = c->d_part;
     }
     hipLaunchKernelGGL((jpar_kernel<ORDER, MODE, GI, POT>), dim3(ngroups, nslice), dim3(NT), 0, c->stream, a);
     hipError_t e = hipGetLastError();
@@ -433,6 +475,13 @@ int hh_create(int order, int nbody, int variant, int device, hh_ctx **out)
     }
     if (e != hipSuccess) return bail("pad", e);
     if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return bail("sync", e);
+    // the device's copy of the static scratch, looked up once per device
+    // with that device current (it is)
+    if (device < HH_MAX_DEVICES && !g_scratch_ptr[device].load()) {
+        void *sp = nullptr;
+        if ((e = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch))) != hipSuccess) return bail("scratch address", e);
+        g_scratch_ptr[device].store((double *)sp);
+    }
     c->dirty_hi.store(nbody);
     *out = c;
     return HH_OK;
@@ -508,6 +557,32 @@ int hh_force_first_nact(hh_ctx *c, int nact, double eps2, double *force, double 
     if (pot) std::memcpy(pot, c->h_pot, sizeof(double) * nact);
     return HH_OK;
 }
+
+int hh_force_first_nact_static(hh_ctx *c, int nact, double eps2, double *force, double *pot)
+{
+    int r = check_ctx(c);
+    if (r) return r;
+    if (nact < 0 || nact > c->nbody) return fail(c, HH_E_ARG, "nact out of range");
+    if (nact > 0 && !force) return fail(c, HH_E_ARG, "force is null");
+    if (nact == 0) return HH_OK;
+    double *sp = c->device < HH_MAX_DEVICES ? g_scratch_ptr[c->device].load() : nullptr;
+    if (!sp) return fail(c, HH_E_STATE, "no static scratch address for this device");
+    c->static_part = sp;
+    c->static_cap = HH_SCRATCH_DOUBLES;
+    r = launch_force(c, nact, eps2, pot != nullptr);
+    c->static_part = nullptr;
+    c->static_cap = 0;
+    if (r) return r;
+    const size_t F = f_words(c->order);
+    HIPCHK(c, hipMemcpyAsync(c->h_force, c->d_force, sizeof(double) * F * nact, hipMemcpyDeviceToHost, c->stream));
+    if (pot) HIPCHK(c, hipMemcpyAsync(c->h_pot, c->d_pot, sizeof(double) * nact, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::memcpy(force, c->h_force, sizeof(double) * F * nact);
+    if (pot) std::memcpy(pot, c->h_pot, sizeof(double) * nact);
+    return HH_OK;
+}
+
+long long hh_static_scratch_doubles(void) { return (long long)HH_SCRATCH_DOUBLES; }
 
 int hh_potential(hh_ctx *c, double eps2, double *pot)
 {

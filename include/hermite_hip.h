@@ -113,6 +113,22 @@ int hh_predict_all(hh_ctx *ctx, double tsys);
  * over the predicted positions (an output the reference Force lacks). */
 int hh_force_first_nact(hh_ctx *ctx, int nact, double eps2, double *force, double *pot);
 
+/* No counterpart in the reference.  hh_force_first_nact for callers that
+ * cannot allocate device memory during the call (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): the same
+ * forces, but the j-slice partial sums live in a static device array of
+ * hh_static_scratch_doubles() doubles (32 MiB, held by every GPU the
+ * library runs on until the process ends) instead of a buffer that grows
+ * with hipMalloc; the call allocates nothing.  The j-slice count of the
+ * launch is capped to what the scratch holds (so for large nact the
+ * summation order, and the last bits, can differ from hh_force_first_nact's;
+ * with the same slice count, hh_set_tuning, the results are identical).
+ * Calls that overlap on one GPU (contexts on different threads or streams)
+ * share that GPU's scratch and race: serialise them.  Returns HH_E_LIMIT if
+ * the partial sums would not fit. */
+int hh_force_first_nact_static(hh_ctx *ctx, int nact, double eps2, double *force, double *pot);
+long long hh_static_scratch_doubles(void);
+
 /* calc_potential(eps2, potbuf): SRC/pot-titan.hu:131-171.  phi_i over the
  * corrected (set_jp) positions of all nbody particles, self excluded. */
 int hh_potential(hh_ctx *ctx, double eps2, double *pot);
