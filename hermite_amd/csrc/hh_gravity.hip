@@ -267,7 +267,7 @@ static int launch_tile_ni(hh_ctx *c, const Geo &g, const double *src, int ni, do
 }
 
 // Default FP32 partial-sum length of the SPLIT body.
-static constexpr int DEFAULT_FOLD = 8;
+static constexpr int DEFAULT_FOLD = 16;
 
 template <int ORDER, int MODE, bool POT>
 static int launch_tile(hh_ctx *c, const Geo &g, const double *src, int ni, double eps2, double *fout, double *pout)

@@ -343,12 +343,58 @@ __device__ __forceinline__ void body_fp64(const double *p, const double *q, doub
 // v_rsq_f32 seed with one Newton step (relative error <= 1.5 h^2 ~ 2^-44,
 // h = 1 - r^2 y0^2), FP64 m r^-3.  The rest of the chain rule runs in FP32:
 // relative velocity, acceleration and jerk from the FP32-rounded
-// predictions, relative position and the scalars rounded from FP64.  Force
-// words [split_lo, F) go to the FP32 partial sums Af, which the caller folds
-// into FP64 every few j; everything else goes to the FP64 accumulators A.
+// predictions (the split record below), relative position and the scalars
+// rounded from FP64.  Force words [split_lo, F) go to the FP32 partial sums
+// Af, which the caller folds into FP64 every few j; everything else goes to
+// the FP64 accumulators A.
 __host__ __device__ constexpr int split_lo(int o) { return o == 8 ? 6 : 3; }
 __host__ __device__ constexpr int n_f32(int o) { return (o == 4 || o == 6 || o == 8) ? f_words(o) - split_lo(o) : 1; }
 
+// Record of the split body (j in LDS or registers, i in registers): pos[3]
+// and mass in FP64, then the velocity (FP64 at 8th order, where the jerk is
+// FP64 arithmetic), the acceleration and the jerk rounded to FP32, two to a
+// double word: order 4 {x y z m (vx vy) (vz -)}, order 6 {x y z m (vx vy)
+// (vz ax) (ay az) -}, order 8 {x y z m vx vy vz (ax ay) (az jx) (jy jz)}.
+// The FP32 words are rounded once per record instead of once per pair.
+__host__ __device__ constexpr int split_words(int o) { return o == 4 ? 6 : o == 6 ? 8 : 10; }
+
+__device__ __forceinline__ double pack2f(float lo, float hi)
+{
+    return __longlong_as_double((long long)(((unsigned long long)__float_as_uint(hi) << 32) | __float_as_uint(lo)));
+}
+__device__ __forceinline__ float lo_f(double d) { return __uint_as_float((unsigned)__double_as_longlong(d)); }
+__device__ __forceinline__ float hi_f(double d)
+{
+    return __uint_as_float((unsigned)((unsigned long long)__double_as_longlong(d) >> 32));
+}
+
+// Prediction record (pos m vel acc jrk, FP64) -> split record.
+template <int ORDER>
+__device__ __forceinline__ void to_split(const double *r, double *s)
+{
+    s[0] = r[0];
+    s[1] = r[1];
+    s[2] = r[2];
+    s[3] = r[3];
+    if constexpr (ORDER == 4) {
+        s[4] = pack2f((float)r[4], (float)r[5]);
+        s[5] = pack2f((float)r[6], 0.0f);
+    } else if constexpr (ORDER == 6) {
+        s[4] = pack2f((float)r[4], (float)r[5]);
+        s[5] = pack2f((float)r[6], (float)r[7]);
+        s[6] = pack2f((float)r[8], (float)r[9]);
+        s[7] = 0.0;
+    } else {
+        s[4] = r[4];
+        s[5] = r[5];
+        s[6] = r[6];
+        s[7] = pack2f((float)r[7], (float)r[8]);
+        s[8] = pack2f((float)r[9], (float)r[10]);
+        s[9] = pack2f((float)r[11], (float)r[12]);
+    }
+}
+
+// p, q: split records of the i and the j particle.
 template <int ORDER, bool POT>
 __device__ __forceinline__ void body_split(const double *p, const double *q, double eps2, bool self, double *A, float *Af)
 {
@@ -369,15 +415,15 @@ __device__ __forceinline__ void body_split(const double *p, const double *q, dou
     const float dxf = (float)dx, dyf = (float)dy, dzf = (float)dz;
     const float ri2f = (float)rinv2, mr3f = (float)mr3;
     if constexpr (ORDER == 4) {
-        const float dvx = (float)q[4] - (float)p[4], dvy = (float)q[5] - (float)p[5], dvz = (float)q[6] - (float)p[6];
+        const float dvx = lo_f(q[4]) - lo_f(p[4]), dvy = hi_f(q[4]) - hi_f(p[4]), dvz = lo_f(q[5]) - lo_f(p[5]);
         const float drdv = __builtin_fmaf(dzf, dvz, __builtin_fmaf(dyf, dvy, dxf * dvx));
         const float a3 = -3.0f * (drdv * ri2f);
         Af[0] = __builtin_fmaf(mr3f, __builtin_fmaf(a3, dxf, dvx), Af[0]);
         Af[1] = __builtin_fmaf(mr3f, __builtin_fmaf(a3, dyf, dvy), Af[1]);
         Af[2] = __builtin_fmaf(mr3f, __builtin_fmaf(a3, dzf, dvz), Af[2]);
     } else if constexpr (ORDER == 6) {
-        const float dvx = (float)q[4] - (float)p[4], dvy = (float)q[5] - (float)p[5], dvz = (float)q[6] - (float)p[6];
-        const float dax = (float)q[7] - (float)p[7], day = (float)q[8] - (float)p[8], daz = (float)q[9] - (float)p[9];
+        const float dvx = lo_f(q[4]) - lo_f(p[4]), dvy = hi_f(q[4]) - hi_f(p[4]), dvz = lo_f(q[5]) - lo_f(p[5]);
+        const float dax = hi_f(q[5]) - hi_f(p[5]), day = lo_f(q[6]) - lo_f(p[6]), daz = hi_f(q[6]) - hi_f(p[6]);
         const float drdv = __builtin_fmaf(dzf, dvz, __builtin_fmaf(dyf, dvy, dxf * dvx));
         const float alpha = drdv * ri2f;
         const float s = __builtin_fmaf(dzf, daz, __builtin_fmaf(dyf, day, __builtin_fmaf(dxf, dax,
@@ -404,8 +450,8 @@ __device__ __forceinline__ void body_split(const double *p, const double *q, dou
         A[5] = __builtin_fma(mr3, tz, A[5]);
         const float dvxf = (float)dvx, dvyf = (float)dvy, dvzf = (float)dvz;
         const float txf = (float)tx, tyf = (float)ty, tzf = (float)tz, alphaf = (float)alpha;
-        const float dax = (float)q[7] - (float)p[7], day = (float)q[8] - (float)p[8], daz = (float)q[9] - (float)p[9];
-        const float djx = (float)q[10] - (float)p[10], djy = (float)q[11] - (float)p[11], djz = (float)q[12] - (float)p[12];
+        const float dax = lo_f(q[7]) - lo_f(p[7]), day = hi_f(q[7]) - hi_f(p[7]), daz = lo_f(q[8]) - lo_f(p[8]);
+        const float djx = hi_f(q[8]) - hi_f(p[8]), djy = lo_f(q[9]) - lo_f(p[9]), djz = hi_f(q[9]) - hi_f(p[9]);
         const float s = __builtin_fmaf(dzf, daz, __builtin_fmaf(dyf, day, __builtin_fmaf(dxf, dax,
                         __builtin_fmaf(dvzf, dvzf, __builtin_fmaf(dvyf, dvyf, dvxf * dvxf)))));
         const float beta = __builtin_fmaf(s, ri2f, alphaf * alphaf);
@@ -463,6 +509,9 @@ __global__ __launch_bounds__(NT) void tile_kernel(TileArgs a)
     constexpr int NC = n_acc(ORDER, POT);
     // i words actually used: pos (+ vel, acc, jrk); mass not needed
     constexpr int QI = ORDER == 0 ? 3 : Q;  // order 1: posH, mass (unused), posL
+    // the split body takes split records (FP32 words packed): LDS stride LS
+    constexpr bool SR = MODE == SPLIT && ORDER > 1;
+    constexpr int LS = SR ? split_words(ORDER) : L;
     __shared__ double sj[TJ * L];
 
     const int tid = threadIdx.x;
@@ -477,6 +526,12 @@ __global__ __launch_bounds__(NT) void tile_kernel(TileArgs a)
         const int idx = ii[k] < a.nact ? ii[k] : a.nact - 1;
 #pragma unroll
         for (int w = 0; w < QI; w++) pi[k][w] = a.src[(size_t)src_word(ORDER, w) * npad + idx];
+        if constexpr (SR) {
+            double t[LS];
+            to_split<ORDER>(pi[k], t);
+#pragma unroll
+            for (int w = 0; w < LS; w++) pi[k][w] = t[w];
+        }
     }
     double A[NI][NC];
 #pragma unroll
@@ -497,8 +552,15 @@ __global__ __launch_bounds__(NT) void tile_kernel(TileArgs a)
 
     for (int t = t0; t < t1; t++) {
         __syncthreads();
+        if constexpr (SR) {
+            double t[LS];
+            to_split<ORDER>(jr, t);
 #pragma unroll
-        for (int w = 0; w < Q; w++) sj[tid * L + w] = jr[w];
+            for (int w = 0; w < LS; w++) sj[tid * LS + w] = t[w];
+        } else {
+#pragma unroll
+            for (int w = 0; w < Q; w++) sj[tid * L + w] = jr[w];
+        }
         __syncthreads();
         if (t + 1 < t1) {
 #pragma unroll
@@ -536,9 +598,9 @@ __global__ __launch_bounds__(NT) void tile_kernel(TileArgs a)
                     for (int c = 0; c < n_f32(ORDER); c++) Af[k][c] = 0.0f;
 #pragma unroll 2
                 for (int jj = jb; jj < jb + FOLD; jj++) {
-                    double q[L];
+                    double q[LS];
 #pragma unroll
-                    for (int w = 0; w < L; w++) q[w] = sj[jj * L + w];
+                    for (int w = 0; w < LS; w++) q[w] = sj[jj * LS + w];
 #pragma unroll
                     for (int k = 0; k < NI; k++)
                         body_split<ORDER, POT>(pi[k], q, a.eps2, jbase + jj == ii[k], A[k], Af[k]);
@@ -659,6 +721,12 @@ __global__ __launch_bounds__(NT) void jpar_kernel(JParArgs a)
         const int idx = ii[g] < a.nact ? ii[g] : a.nact - 1;
 #pragma unroll
         for (int w = 0; w < Q; w++) pi[g][w] = a.src[(size_t)w * npad + idx];
+        if constexpr (MODE == SPLIT) {
+            double t[split_words(ORDER)];
+            to_split<ORDER>(pi[g], t);
+#pragma unroll
+            for (int w = 0; w < split_words(ORDER); w++) pi[g][w] = t[w];
+        }
     }
     double A[GI][NC];
 #pragma unroll
@@ -678,6 +746,12 @@ __global__ __launch_bounds__(NT) void jpar_kernel(JParArgs a)
         double q[Q];
 #pragma unroll
         for (int w = 0; w < Q; w++) q[w] = a.src[(size_t)w * npad + j];
+        if constexpr (MODE == SPLIT) {
+            double t[split_words(ORDER)];
+            to_split<ORDER>(q, t);
+#pragma unroll
+            for (int w = 0; w < split_words(ORDER); w++) q[w] = t[w];
+        }
 #pragma unroll
         for (int g = 0; g < GI; g++) {
             if constexpr (MODE == FP64) body_fp64<ORDER, POT>(pi[g], q, a.eps2, j == ii[g], 0, A[g]);

@@ -613,8 +613,11 @@ __device__ __forceinline__ void r_force_body(const RArgs &a)
     constexpr int F = f_words(ORDER);
     constexpr int NC = F;
     constexpr int V = R_GI * NC;
+    // the split body takes split records (FP32 words packed): LDS stride LS
+    constexpr bool SR = MODE == SPLIT;
+    constexpr int LS = SR ? split_words(ORDER) : L;
     constexpr int NW = NT / 64;
-    constexpr int FOLD = 8;
+    constexpr int FOLD = 16;  // j per FP32 partial sum of the split body
     __shared__ double sj[TJ * L];
     __shared__ double red[NW][V];
     __shared__ int s_key[R_CLU_LDS];
@@ -663,6 +666,12 @@ __device__ __forceinline__ void r_force_body(const RArgs &a)
                 const int ip = c.off + min(ii[k], c.nact - 1);
 #pragma unroll
                 for (int w = 0; w < Q; w++) pi[k][w] = a.ipr[(size_t)w * npad + ip];
+                if constexpr (SR) {
+                    double t[LS];
+                    to_split<ORDER>(pi[k], t);
+#pragma unroll
+                    for (int w = 0; w < LS; w++) pi[k][w] = t[w];
+                }
             }
             // a wave whose lanes are all past the active list only helps
             // stage the tiles; its SIMD slots go to the other waves
@@ -680,8 +689,15 @@ __device__ __forceinline__ void r_force_body(const RArgs &a)
             for (int w = 0; w < Q; w++) jr[w] = src[(size_t)w * npad + (tb + t0) * TJ + tid];
             for (int t = t0; t < t1; t++) {
                 __syncthreads();
+                if constexpr (SR) {
+                    double tq[LS];
+                    to_split<ORDER>(jr, tq);
 #pragma unroll
-                for (int w = 0; w < Q; w++) sj[tid * L + w] = jr[w];
+                    for (int w = 0; w < LS; w++) sj[tid * LS + w] = tq[w];
+                } else {
+#pragma unroll
+                    for (int w = 0; w < Q; w++) sj[tid * L + w] = jr[w];
+                }
                 __syncthreads();
                 if (t + 1 < t1) {
 #pragma unroll
@@ -714,9 +730,9 @@ __device__ __forceinline__ void r_force_body(const RArgs &a)
                             for (int q = 0; q < n_f32(ORDER); q++) Af[k][q] = 0.0f;
 #pragma unroll 2
                         for (int jj = jb; jj < jb + FOLD; jj++) {
-                            double q[L];
+                            double q[LS];
 #pragma unroll
-                            for (int w = 0; w < L; w++) q[w] = sj[jj * L + w];
+                            for (int w = 0; w < LS; w++) q[w] = sj[jj * LS + w];
 #pragma unroll
                             for (int k = 0; k < NI; k++) body_split<ORDER, false>(pi[k], q, eps2, false, A[k], Af[k]);
                         }
@@ -763,6 +779,12 @@ __device__ __forceinline__ void r_force_body(const RArgs &a)
                 const int ip = c.off + min(grp * R_GI + g, c.nact - 1);
 #pragma unroll
                 for (int w = 0; w < Q; w++) pi[g][w] = a.ipr[(size_t)w * npad + ip];
+                if constexpr (SR) {
+                    double t[LS];
+                    to_split<ORDER>(pi[g], t);
+#pragma unroll
+                    for (int w = 0; w < LS; w++) pi[g][w] = t[w];
+                }
             }
             double A[R_GI][NC];
 #pragma unroll
@@ -781,6 +803,12 @@ __device__ __forceinline__ void r_force_body(const RArgs &a)
                 double q[Q];
 #pragma unroll
                 for (int w = 0; w < Q; w++) q[w] = src[(size_t)w * npad + j];
+                if constexpr (SR) {
+                    double t[LS];
+                    to_split<ORDER>(q, t);
+#pragma unroll
+                    for (int w = 0; w < LS; w++) q[w] = t[w];
+                }
 #pragma unroll
                 for (int g = 0; g < R_GI; g++) {
                     if constexpr (MODE == FP64) body_fp64<ORDER, false>(pi[g], q, eps2, false, 0, A[g]);

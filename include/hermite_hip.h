@@ -47,8 +47,11 @@ typedef struct hh_ctx hh_ctx;
 enum {
     HH_FP64 = 0,  /* all-FP64 interaction, FMA + FP32-seeded rsqrt refined to
                      FP64; the drop-in for the TITAN back end */
-    HH_SPLIT = 1, /* FP64 relative quantities, FP32 interaction body, FP32
-                     per-tile partial sums folded into FP64 accumulators */
+    HH_SPLIT = 1, /* per-family precision: the acceleration (8th order:
+                     and the jerk) and the potential in FP64 arithmetic;
+                     the higher families' chain rule in FP32 on FP64
+                     relative positions, FP32 partial sums of 16 j folded
+                     into FP64 accumulators */
     HH_EXACT = 2  /* IEEE FP64 with the generic back end's exact operation
                      order (SRC/hermite{4,6,8}.h generic Gravity): results
                      bit-identical to the reference CPU build */
