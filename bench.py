@@ -357,9 +357,9 @@ def kernel_rate(ft, order):
 
 
 def opmix_from_profile(order):
-    """FP64 and FP32 shares of the split force kernel's VALU arithmetic
-    instructions, from the rocprofv3 counter summary of this order, if
-    recorded: {"f64": share, "f32": share}."""
+    """Lane instructions per interaction of the split force kernel, from the
+    rocprofv3 counter summary of this order if recorded: {"f64": FP64-rate
+    instructions (FP64 arithmetic and conversions), "f32": FP32 ones}."""
     path = os.path.join(ROOT, "profiles", f"r_force_kernel_{order}th_split_opmix.md")
     try:
         with open(path) as fh:
@@ -372,17 +372,19 @@ def opmix_from_profile(order):
 
 
 def split_roofline(w, avg_ms, achieved, launches):
-    """Roofline of r_force_kernel<ORDER,SPLIT>: its conventional TF/s against
-    the FP64 vector peak, the FP32 vector peak, and the blend of the two that
-    its measured instruction mix allows (1 / (s64/78.6 + s32/157.3) for
-    FP64 and FP32 VALU instruction shares s64, s32)."""
+    """Roofline of r_force_kernel<ORDER,SPLIT>: its conventional TF/s
+    (60/97/144 flops per interaction) against the FP64 vector peak, the FP32
+    vector peak, and the blend of the two that its measured instruction mix
+    allows: with n64 FP64-rate and n32 FP32 lane instructions per
+    interaction, issued at 78.6/2 and 157.3/2 T per second, the blended peak
+    is flops / (n64 / 39.3 + n32 / 78.65) TF/s."""
     mix = opmix_from_profile(w.order)
     out = {"bound": "valu", "kernel": f"r_force_kernel<{w.order},SPLIT>", "achieved": achieved,
            "unit": "TFLOP/s", "avg_launch_ms": avg_ms, "launches": launches,
            "peak_fp64": PEAK_FP64_TFLOPS, "frac_fp64": achieved / PEAK_FP64_TFLOPS,
            "peak_fp32": PEAK_FP32_TFLOPS, "frac_fp32": achieved / PEAK_FP32_TFLOPS, "opmix": mix}
     if mix:
-        blend = 1.0 / (mix["f64"] / PEAK_FP64_TFLOPS + mix["f32"] / PEAK_FP32_TFLOPS)
+        blend = H.FLOPS[w.order] / (mix["f64"] / (PEAK_FP64_TFLOPS / 2) + mix["f32"] / (PEAK_FP32_TFLOPS / 2))
         out.update({"peak_blended": blend, "frac_blended": achieved / blend})
     return out
 
