@@ -246,16 +246,42 @@ def traffic_from_profile(w):
     return None
 
 
-def _cpu_model():
-    cpu = platform.processor() or platform.machine()
+def _cpu_info():
+    """lscpu model, sockets, cores per socket and threads per core, and the
+    CPUs this process may run on."""
+    info = {"model": platform.processor() or platform.machine()}
+    keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+            "Thread(s) per core": "threads_per_core", "CPU(s)": "cpus"}
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
         for ln in out.splitlines():
-            if ln.startswith("Model name"):
-                cpu = ln.split(":", 1)[1].strip()
+            k, _, v = ln.partition(":")
+            if k.strip() in keys:
+                v = v.strip()
+                info[keys[k.strip()]] = int(v) if v.isdigit() else v
     except OSError:
         pass
-    return cpu
+    try:
+        info["cpus_allowed"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        pass
+    return info
+
+
+def _cpu_threads(info):
+    """Threads for the CPU baseline: min(physical cores, 64) (the reference's
+    MAXTHREAD, SRC/hermite6-avx.h:8), further capped by the CPU share this
+    job is given (OMP_NUM_THREADS, which the GPU box sets to its 16 cores
+    per GPU, and the CPUs the process may run on)."""
+    phys = info.get("sockets", 0) * info.get("cores_per_socket", 0) if isinstance(info.get("sockets"), int) \
+        and isinstance(info.get("cores_per_socket"), int) else 0
+    n = min(phys or (os.cpu_count() or 1), 64)
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    if share:
+        n = min(n, share)
+    if info.get("cpus_allowed"):
+        n = min(n, info["cpus_allowed"])
+    return max(1, n)
 
 
 def _reps_for(secs, target_s=10.0, most=2000):
@@ -275,8 +301,12 @@ def cpu_baseline(w, states, cpu_tend, budget_s=20.0):
         ref = oracle.RefAVX(w.order)
     except (ImportError, FileNotFoundError):
         return None
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
-    cpu = _cpu_model()
+    info = _cpu_info()
+    threads = _cpu_threads(info)
+    cpu = info["model"]
+    # thread placement of the reference's OpenMP runs (child processes)
+    os.environ["OMP_PLACES"] = "cores"
+    os.environ["OMP_PROC_BIND"] = "close"
     if len(states) == 1 and len(states[0][3]) <= 65536:
         P_raw, eps2 = states[0][3], states[0][1]
         info = ref.run(P_raw, eps2, w.eta, w.eta_s, w.dtmax, cpu_tend, threads=threads)
@@ -323,7 +353,9 @@ def cpu_baseline(w, states, cpu_tend, budget_s=20.0):
                   f"a multiple of 4, each t=0->{cpu_tend:g}" + (f" run {reps} times" if reps > 1 else "")
                   + f" ({secs:.1f} s), {cpu}")
     out = {"value": inter / secs, "unit": "interactions/s", "cores": used, "kind": "reference",
-           "sample": sample, "gflops": inter / secs * H.FLOPS[w.order] * 1e-9}
+           "sample": sample, "gflops": inter / secs * H.FLOPS[w.order] * 1e-9,
+           "per_core": inter / secs / max(used, 1), "cpu": info,
+           "placement": "OMP_PLACES=cores OMP_PROC_BIND=close"}
     if len(states) == 1 and len(states[0][0]) <= 65536 and len(states[0][0]) % 4 == 0:
         # the reference's "Init N square" evaluation: all N on all N, once
         P0, eps2 = states[0][0], states[0][1]

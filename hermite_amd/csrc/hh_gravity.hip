@@ -8,6 +8,7 @@
 //                         fused scatter + predict kernel over all slots
 //   hh_force_first_nact   force kernel, D2H of nact Force records
 #include "hh_kernels.hpp"
+#include "hh_jstore.hpp"
 #include "../../include/hermite_hip.h"
 
 #include <algorithm>
@@ -501,37 +502,17 @@ double *hh_jstore(hh_ctx *c) { return c ? c->h_jp : nullptr; }
 int hh_mark_dirty(hh_ctx *c, int lo, int hi)
 {
     if (!c || lo < 0 || hi > c->nbody || lo > hi) return HH_E_ARG;
-    int cur = c->dirty_hi.load(std::memory_order_relaxed);
-    while (cur < hi && !c->dirty_hi.compare_exchange_weak(cur, hi, std::memory_order_relaxed)) {
-    }
+    dirty_raise(c->dirty_hi, hi);
     return HH_OK;
 }
 
-// set_jp: SRC/hermite6-titan.h:70-97 (field copy into the pinned j-store).
-// The double-double Particle (SRC/hermite8dd.h:44-78) has posL after posH;
-// its j-store record is the 8th-order one followed by posL.
+// set_jp: SRC/hermite6-titan.h:70-97 (hh_jstore.hpp).
 int hh_set_jp(hh_ctx *c, int addr, const double *p)
 {
     if (!c || !p || addr < 0 || addr >= c->nbody) return HH_E_ARG;
-    const int W = jp_words(c->order);
-    double *d = c->h_jp + (size_t)addr * W;
-    if (c->order == HH_ORDER_8DD) {
-        d[0] = p[4];  d[1] = p[5];  d[2] = p[6];     // posH
-        d[3] = p[1];                                 // mass
-        d[4] = p[10]; d[5] = p[11]; d[6] = p[12];    // vel
-        d[7] = p[2];                                 // tlast
-        for (int w = 0; w < 18; w++) d[8 + w] = p[13 + w];  // acc jrk snp crk d4a d5a
-        d[26] = p[7]; d[27] = p[8]; d[28] = p[9];    // posL
-        return hh_mark_dirty(c, 0, addr + 1);
-    }
-    d[0] = p[4];  d[1] = p[5];  d[2] = p[6];   // pos
-    d[3] = p[1];                               // mass
-    d[4] = p[7];  d[5] = p[8];  d[6] = p[9];   // vel
-    d[7] = p[2];                               // tlast
-    for (int w = 0; w < 6; w++) d[8 + w] = p[10 + w];                     // acc, jrk
-    if (c->order >= 6) for (int w = 0; w < 6; w++) d[14 + w] = p[16 + w];  // snp, crk
-    if (c->order == 8) for (int w = 0; w < 6; w++) d[20 + w] = p[22 + w];  // d4a, d5a
-    return hh_mark_dirty(c, 0, addr + 1);
+    jstore_write(c->order, c->h_jp, addr, p);
+    dirty_raise(c->dirty_hi, addr + 1);
+    return HH_OK;
 }
 
 int hh_predict_all(hh_ctx *c, double tsys)

@@ -5,7 +5,8 @@
 // Compiled with -ffp-contract=off and no -march flags, so that with the
 // reference's seed (19640916) and truncation radius (100) the sample is
 // bit-identical to the snapshot gen-plum writes (checked by
-// tests/test_inputs.py against a gen-plum built from its source).
+// tests/test_golden.py::test_generator_reproduces_gen_plum against a
+// gen-plum built from its source).
 #include <cmath>
 #include <cstdint>
 #include <cstring>

@@ -357,6 +357,9 @@ __host__ __device__ constexpr int n_f32(int o) { return (o == 4 || o == 6 || o =
 // (vz ax) (ay az) -}, order 8 {x y z m vx vy vz (ax ay) (az jx) (jy jz)}.
 // The FP32 words are rounded once per record instead of once per pair.
 __host__ __device__ constexpr int split_words(int o) { return o == 4 ? 6 : o == 6 ? 8 : 10; }
+// LDS stride of split records: an odd number of 16-byte words, so the
+// staging stores of consecutive lanes fall in different banks
+__host__ __device__ constexpr int split_stride(int o) { return o == 6 ? 10 : split_words(o); }
 
 __device__ __forceinline__ double pack2f(float lo, float hi)
 {
@@ -512,6 +515,7 @@ __global__ __launch_bounds__(NT) void tile_kernel(TileArgs a)
     // the split body takes split records (FP32 words packed): LDS stride LS
     constexpr bool SR = MODE == SPLIT && ORDER > 1;
     constexpr int LS = SR ? split_words(ORDER) : L;
+    constexpr int LST = SR ? split_stride(ORDER) : L;
     __shared__ double sj[TJ * L];
 
     const int tid = threadIdx.x;
@@ -556,7 +560,7 @@ __global__ __launch_bounds__(NT) void tile_kernel(TileArgs a)
             double t[LS];
             to_split<ORDER>(jr, t);
 #pragma unroll
-            for (int w = 0; w < LS; w++) sj[tid * LS + w] = t[w];
+            for (int w = 0; w < LS; w++) sj[tid * LST + w] = t[w];
         } else {
 #pragma unroll
             for (int w = 0; w < Q; w++) sj[tid * L + w] = jr[w];
@@ -600,7 +604,7 @@ __global__ __launch_bounds__(NT) void tile_kernel(TileArgs a)
                 for (int jj = jb; jj < jb + FOLD; jj++) {
                     double q[LS];
 #pragma unroll
-                    for (int w = 0; w < LS; w++) q[w] = sj[jj * LS + w];
+                    for (int w = 0; w < LS; w++) q[w] = sj[jj * LST + w];
 #pragma unroll
                     for (int k = 0; k < NI; k++)
                         body_split<ORDER, POT>(pi[k], q, a.eps2, jbase + jj == ii[k], A[k], Af[k]);

@@ -616,6 +616,7 @@ __device__ __forceinline__ void r_force_body(const RArgs &a)
     // the split body takes split records (FP32 words packed): LDS stride LS
     constexpr bool SR = MODE == SPLIT;
     constexpr int LS = SR ? split_words(ORDER) : L;
+    constexpr int LST = SR ? split_stride(ORDER) : L;
     constexpr int NW = NT / 64;
     constexpr int FOLD = 16;  // j per FP32 partial sum of the split body
     __shared__ double sj[TJ * L];
@@ -693,7 +694,7 @@ __device__ __forceinline__ void r_force_body(const RArgs &a)
                     double tq[LS];
                     to_split<ORDER>(jr, tq);
 #pragma unroll
-                    for (int w = 0; w < LS; w++) sj[tid * LS + w] = tq[w];
+                    for (int w = 0; w < LS; w++) sj[tid * LST + w] = tq[w];
                 } else {
 #pragma unroll
                     for (int w = 0; w < Q; w++) sj[tid * L + w] = jr[w];
@@ -732,7 +733,7 @@ __device__ __forceinline__ void r_force_body(const RArgs &a)
                         for (int jj = jb; jj < jb + FOLD; jj++) {
                             double q[LS];
 #pragma unroll
-                            for (int w = 0; w < LS; w++) q[w] = sj[jj * LS + w];
+                            for (int w = 0; w < LS; w++) q[w] = sj[jj * LST + w];
 #pragma unroll
                             for (int k = 0; k < NI; k++) body_split<ORDER, false>(pi[k], q, eps2, false, A[k], Af[k]);
                         }
