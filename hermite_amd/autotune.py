@@ -139,6 +139,111 @@ def write_text(rows, path):
             f.write("%d %d %d %d\n" % tuple(r))
 
 
+# ----------------------------------------------------------- resident plan
+# Work split of the GPU-resident block step (hhr_set_plan, hhr_set_split_waves):
+# jpar_max, smax, task_target, jpar_target, wj4_max, wj2_max.
+PLAN_DEFAULT = (64, 256, 8192, 256, 8192, 16384)
+PLAN_NAMES = ("jpar_max", "smax", "task_target", "jpar_target", "wj4_max", "wj2_max")
+PLAN_CHOICES = ((32, 64, 128), (128, 256, 512), (4096, 8192), (128, 256, 512), (2048, 8192, 16384),
+                (8192, 16384, 32768))
+PLAN_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning", "mi355x_plan.json")
+# span timed per candidate: whole dt_max intervals, or a block-step limit
+PLAN_SPAN = {"c1": (1.0, 0), "c2": (0.25, 0), "c3": (1 / 32, 0), "c4": (0.25, 0), "c5": (1 / 16, 384)}
+
+
+def plan_key(order, variant, n_total, nclu):
+    return f"{order}/{variant}/{n_total}/{nclu}"
+
+
+def _resident_states(w):
+    import hermite_amd as H
+    from hermite_amd import workloads as W
+    out = []
+    for c in w.clusters:
+        s = H.System(w.order, W.cluster_state(w.order, c), 0.0, c.eps ** 2, w.eta, w.eta_s, w.dtmax, w.dtmin,
+                     variant="fp64")
+        try:
+            s.init_step()
+            out.append((s.ptcl(), c.eps ** 2))
+        finally:
+            s.close()
+    return out
+
+
+def _time_plan(w, variant, states, plan, span, max_bsteps):
+    """Interactions per second of a resident integration over the span."""
+    import hermite_amd as H
+    r = H.Resident(w.order, variant, [(P, e2, w.eta, w.dtmax, w.dtmin, 0.0) for P, e2 in states])
+    try:
+        r.set_plan(*plan[:4])
+        r.set_split_waves(*plan[4:])
+        if max_bsteps:
+            r.set_step_limit(max_bsteps)
+        t0 = time.perf_counter()
+        try:
+            r.run(span)
+        except H.StepLimitReached:
+            pass
+        wall = time.perf_counter() - t0
+        inter = sum(r.stats(k)["num_step"] * len(states[k][0]) for k in range(len(states)))
+    finally:
+        r.close()
+    return inter / wall
+
+
+def tune_plan(name, variant="fp64", log=None, passes=1):
+    """Coordinate descent over PLAN_CHOICES on workload `name`, starting from
+    PLAN_DEFAULT; each candidate is timed twice (best kept).  Returns the
+    best plan, its rate, the default's rate and every measurement."""
+    from hermite_amd import workloads as W
+    w = W.workload(name)
+    span, maxb = PLAN_SPAN[name]
+    states = _resident_states(w)
+    meas = {}
+
+    def rate(plan):
+        if plan not in meas:
+            meas[plan] = max(_time_plan(w, variant, states, plan, span, maxb) for _ in range(2))
+            if log:
+                print(f"{name} {variant} plan {plan}: {meas[plan]:.4e} int/s", file=log, flush=True)
+        return meas[plan]
+
+    best = PLAN_DEFAULT
+    base = rate(best)
+    for _ in range(passes):
+        for i, choices in enumerate(PLAN_CHOICES):
+            for v in choices:
+                cand = best[:i] + (v,) + best[i + 1:]
+                if cand[4] > cand[5]:
+                    continue
+                if rate(cand) > rate(best):
+                    best = cand
+    return best, rate(best), base, meas, w
+
+
+PLAN_HEADER = os.path.join(os.path.dirname(os.path.abspath(__file__)), "csrc", "hh_plan_table.hpp")
+
+
+def write_plan_header(plans, path=PLAN_HEADER):
+    """The measured plans as the C++ table hhr_create chooses from."""
+    rows = []
+    for k in sorted(plans, key=lambda k: [int(x) if x.isdigit() else x for x in k.split("/")]):
+        order, variant, n, nclu = k.split("/")
+        p = plans[k]["plan"]
+        rows.append(f"    {{{order}, {n}, {nclu}, {{{p['jpar_max']}, {p['smax']}, {p['task_target']}, "
+                    f"{p['jpar_target']}, {p['wj4_max']}, {p['wj2_max']}}}}},  // {plans[k]['workload']}, "
+                    f"{variant}: {plans[k]['gain']:.3f}x the built-in plan")
+    with open(path, "w") as f:
+        f.write("// Resident block-step plans measured on MI355X by\n"
+                "// `python -m hermite_amd.autotune --plan c1 c2 c3 c4 c5` (hermite_amd/tuning/mi355x_plan.json):\n"
+                "// order, total particles, clusters, then hhr_set_plan's and hhr_set_split_waves'\n"
+                "// parameters.  hhr_create takes the row of its order and kind (one cluster or an\n"
+                "// ensemble) nearest in total particles, within a factor of 4.\n"
+                "#pragma once\n\nnamespace hh {\n\nstruct PlanRow {\n    int order, n_total, nclu;\n"
+                "    int v[6];  // jpar_max smax task_target jpar_target wj4_max wj2_max\n};\n\n"
+                "static const PlanRow kPlanTable[] = {\n" + "\n".join(rows) + "\n};\n\n}  // namespace hh\n")
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--orders", type=int, nargs="+", default=[4, 6, 8])
@@ -149,7 +254,30 @@ def main(argv=None):
     ap.add_argument("--device", type=int, default=0)
     ap.add_argument("--max-nact", type=int, default=0,
                     help="largest active count measured (0: N); above it the built-in geometry applies")
+    ap.add_argument("--plan", nargs="+", default=None, metavar="WORKLOAD",
+                    help="instead: tune the resident plan on these workloads (c1 ... c5)")
+    ap.add_argument("--plan-variants", nargs="+", default=["fp64"])
+    ap.add_argument("--plan-out", default=PLAN_PATH)
+    ap.add_argument("--plan-header", default=PLAN_HEADER, help="C++ table written from the plans ('' for none)")
     a = ap.parse_args(argv)
+    if a.plan:
+        plans = {}
+        if os.path.exists(a.plan_out):
+            with open(a.plan_out) as f:
+                plans = json.load(f)
+        for name in a.plan:
+            for variant in a.plan_variants:
+                best, r_best, r_base, meas, w = tune_plan(name, variant, log=sys.stdout)
+                plans[plan_key(w.order, variant, w.n_total, len(w.clusters))] = {
+                    "workload": name, "plan": dict(zip(PLAN_NAMES, best)), "rate": r_best, "default_rate": r_base,
+                    "gain": r_best / r_base}
+                print(f"{name} {variant}: best {best} {r_best:.4e} int/s, default {r_base:.4e} "
+                      f"({r_best / r_base:.3f}x)", flush=True)
+                with open(a.plan_out, "w") as f:
+                    json.dump(plans, f, indent=1, sort_keys=True)
+        if a.plan_header:
+            write_plan_header(plans, a.plan_header)
+        return
     tables = load(a.out) if os.path.exists(a.out) else {}
     allmeas = {}
     t0 = time.time()

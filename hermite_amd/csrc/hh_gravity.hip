@@ -9,6 +9,7 @@
 //   hh_force_first_nact   force kernel, D2H of nact Force records
 #include "hh_kernels.hpp"
 #include "hh_jstore.hpp"
+#include "hh_plan_table.hpp"
 #include "../../include/hermite_hip.h"
 
 #include <algorithm>
@@ -870,6 +871,29 @@ static void r_free(hhr_ctx *c)
     if (c->stream) (void)hipStreamDestroy(c->stream);
 }
 
+// The measured plan of this order and kind (one cluster or an ensemble)
+// nearest in total particles, within a factor of 4 (hh_plan_table.hpp);
+// the built-in plan otherwise.
+static RPlan measured_plan(int order, int nclu, long long n_total)
+{
+    RPlan p;
+    double best = 2.0 + 1e-9;
+    for (const PlanRow &r : kPlanTable) {
+        if (r.order != order || (r.nclu > 1) != (nclu > 1)) continue;
+        const double d = std::fabs(std::log2((double)r.n_total) - std::log2((double)n_total));
+        if (d <= best) {
+            best = d;
+            p.jpar_max = r.v[0];
+            p.smax = r.v[1];
+            p.task_target = r.v[2];
+            p.jpar_target = r.v[3];
+            p.wj4_max = r.v[4];
+            p.wj2_max = r.v[5];
+        }
+    }
+    return p;
+}
+
 extern "C" {
 
 const char *hhr_last_error(const hhr_ctx *c) { return c ? c->err.c_str() : g_hhr_err.c_str(); }
@@ -983,7 +1007,7 @@ int hhr_create(int order, int variant, int device, int nclu, const int *ns, cons
     std::memset(&g, 0, sizeof(g));
     g.nclu = nclu;
     g.exact = variant == HH_EXACT;
-    g.plan = RPlan{};
+    g.plan = measured_plan(order, nclu, (long long)src);  // src: the clusters' particles in total
 
     auto bail = [&](const char *what, hipError_t e) {
         g_hhr_err = std::string(what) + ": " + hipGetErrorString(e);
