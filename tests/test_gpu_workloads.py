@@ -179,3 +179,33 @@ def test_c4_ensemble_sample_equals_clusters_alone():
         assert np.median(de) <= 2 * np.median(de_exact) + 1e-12
         assert de.max() <= 4 * de_exact.max() + 1e-12
         assert nb.min() > 0
+
+
+def test_c5_recipe_energy_vs_oracle_at_small_n():
+    """The c5 recipe (Salpeter masses 0.1-100, eps 1/4096, eta 0.2,
+    dt_max 1/16, dt_min 2^-40) at N = 2048, t = 0 -> 1/16, where the oracle
+    integrates in about a second: the FP64 and split resident runs take the
+    oracle's block steps to 2 % and reach |dE/E| within twice the oracle's
+    (the full-size run's 2.7e-4 comes from close encounters of the massive
+    stars at N = 2^20, which this sample cannot show)."""
+    w = W.workload("c5")
+    c = W.Cluster(2048, 5, 1 / 4096, "salpeter", 5005)
+    eps2 = c.eps ** 2
+    o = O.System(w.order, W.cluster_state(w.order, c), 0.0, eps2, w.eta, w.eta_s, w.dtmax)
+    o.init_step()
+    P0, e0 = o.ptcl(), o.init_energy
+    o.integrate_until_inplace(w.tend)
+    de_o = abs((o.energy() - e0) / e0)
+    for variant in ("fp64", "split"):
+        r = H.Resident(w.order, variant, [(P0, eps2, w.eta, w.dtmax, w.dtmin, 0.0)])
+        try:
+            r.run(w.tend)
+            st = r.stats()
+            de = abs((r.energy() - e0) / e0)
+        finally:
+            r.close()
+        print(f"c5 recipe N=2048 {variant}: {st['num_bstep']} block steps (oracle {o.num_bstep}), |dE/E| {de:.3e} "
+              f"(oracle {de_o:.3e})", flush=True)
+        assert abs(st["num_bstep"] / o.num_bstep - 1) <= 0.02
+        assert st["dtmin_hits"] == 0
+        assert de <= 2 * de_o + 1e-13
