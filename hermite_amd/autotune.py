@@ -224,21 +224,34 @@ def tune_plan(name, variant="fp64", log=None, passes=1):
 PLAN_HEADER = os.path.join(os.path.dirname(os.path.abspath(__file__)), "csrc", "hh_plan_table.hpp")
 
 
+PLAN_MIN_GAIN = 1.02  # below this the built-in plan is kept (run-to-run noise is 1-2 %)
+
+
 def write_plan_header(plans, path=PLAN_HEADER):
-    """The measured plans as the C++ table hhr_create chooses from."""
-    rows = []
+    """The measured plans as the C++ table hhr_create chooses from.  A plan
+    enters the table only if it beat the built-in one by PLAN_MIN_GAIN;
+    the others are listed as comments (measured, built-in plan kept)."""
+    rows, kept = [], []
     for k in sorted(plans, key=lambda k: [int(x) if x.isdigit() else x for x in k.split("/")]):
         order, variant, n, nclu = k.split("/")
-        p = plans[k]["plan"]
-        rows.append(f"    {{{order}, {n}, {nclu}, {{{p['jpar_max']}, {p['smax']}, {p['task_target']}, "
-                    f"{p['jpar_target']}, {p['wj4_max']}, {p['wj2_max']}}}}},  // {plans[k]['workload']}, "
-                    f"{variant}: {plans[k]['gain']:.3f}x the built-in plan")
+        e = plans[k]
+        p = e["plan"]
+        vals = f"{p['jpar_max']}, {p['smax']}, {p['task_target']}, {p['jpar_target']}, {p['wj4_max']}, {p['wj2_max']}"
+        if e["gain"] >= PLAN_MIN_GAIN:
+            rows.append(f"    {{{order}, {n}, {nclu}, {{{vals}}}}},  // {e['workload']}, {variant}: "
+                        f"{e['gain']:.3f}x the built-in plan")
+        else:
+            kept.append(f"//   order {order}, N {n}, {nclu} cluster(s) ({e['workload']}, {variant}): best {{{vals}}} "
+                        f"measured {e['gain']:.3f}x the built-in plan")
+    rows.append("    {0, 0, 0, {0, 0, 0, 0, 0, 0}},  // end of table")
     with open(path, "w") as f:
         f.write("// Resident block-step plans measured on MI355X by\n"
                 "// `python -m hermite_amd.autotune --plan c1 c2 c3 c4 c5` (hermite_amd/tuning/mi355x_plan.json):\n"
                 "// order, total particles, clusters, then hhr_set_plan's and hhr_set_split_waves'\n"
                 "// parameters.  hhr_create takes the row of its order and kind (one cluster or an\n"
-                "// ensemble) nearest in total particles, within a factor of 4.\n"
+                f"// ensemble) nearest in total particles, within a factor of 4.  Only plans that beat\n"
+                f"// the built-in one by {PLAN_MIN_GAIN:.2f}x or more are rows; for these the built-in plan\n"
+                "// was kept (within run-to-run noise of the best found):\n" + "\n".join(kept) + "\n"
                 "#pragma once\n\nnamespace hh {\n\nstruct PlanRow {\n    int order, n_total, nclu;\n"
                 "    int v[6];  // jpar_max smax task_target jpar_target wj4_max wj2_max\n};\n\n"
                 "static const PlanRow kPlanTable[] = {\n" + "\n".join(rows) + "\n};\n\n}  // namespace hh\n")

@@ -2,7 +2,14 @@
 // `python -m hermite_amd.autotune --plan c1 c2 c3 c4 c5` (hermite_amd/tuning/mi355x_plan.json):
 // order, total particles, clusters, then hhr_set_plan's and hhr_set_split_waves'
 // parameters.  hhr_create takes the row of its order and kind (one cluster or an
-// ensemble) nearest in total particles, within a factor of 4.
+// ensemble) nearest in total particles, within a factor of 4.  Only plans that beat
+// the built-in one by 1.02x or more are rows; for these the built-in plan
+// was kept (within run-to-run noise of the best found):
+//   order 4, N 1024, 1 cluster(s) (c1, fp64): best {128, 256, 8192, 256, 8192, 8192} measured 1.008x the built-in plan
+//   order 4, N 549993, 256 cluster(s) (c4, fp64): best {64, 128, 8192, 512, 8192, 32768} measured 1.004x the built-in plan
+//   order 6, N 65536, 1 cluster(s) (c2, fp64): best {128, 512, 8192, 256, 8192, 16384} measured 1.013x the built-in plan
+//   order 6, N 1048576, 1 cluster(s) (c5, fp64): best {64, 128, 8192, 256, 8192, 16384} measured 1.002x the built-in plan
+//   order 8, N 262144, 1 cluster(s) (c3, fp64): best {128, 256, 8192, 256, 16384, 16384} measured 1.002x the built-in plan
 #pragma once
 
 namespace hh {
@@ -13,11 +20,7 @@ struct PlanRow {
 };
 
 static const PlanRow kPlanTable[] = {
-    {4, 1024, 1, {128, 256, 8192, 256, 8192, 8192}},  // c1, fp64: 1.008x the built-in plan
-    {4, 549993, 256, {64, 128, 8192, 512, 8192, 32768}},  // c4, fp64: 1.004x the built-in plan
-    {6, 65536, 1, {128, 512, 8192, 256, 8192, 16384}},  // c2, fp64: 1.013x the built-in plan
-    {6, 1048576, 1, {64, 128, 8192, 256, 8192, 16384}},  // c5, fp64: 1.002x the built-in plan
-    {8, 262144, 1, {128, 256, 8192, 256, 16384, 16384}},  // c3, fp64: 1.002x the built-in plan
+    {0, 0, 0, {0, 0, 0, 0, 0, 0}},  // end of table
 };
 
 }  // namespace hh

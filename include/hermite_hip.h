@@ -232,12 +232,14 @@ int hhr_set_step_limit(hhr_ctx *ctx, long long max_block_steps);
  * i-parallel groups of 512 i (128 or 256 when tiles are split over waves,
  * hhr_set_split_waves) with at most smax j-slices; a block step aims at
  * task_target i-parallel (jpar_target j-parallel) tasks over all clusters
- * (both at most 8192).  hhr_create starts from the plan measured on MI355X
- * for the same order and kind (one cluster or an ensemble) nearest in total
- * particles, within a factor of 4 (hermite_amd/tuning/mi355x_plan.json, for
- * 4th order N = 1024 and a 256-cluster ensemble of 550k particles, 6th order
- * N = 2^16 and 2^20, 8th order N = 2^18), otherwise from 64, 256, 8192, 256
- * (split waves 8192, 16384). */
+ * (both at most 8192).  hhr_create starts from the measured plan of the
+ * same order and kind (one cluster or an ensemble) nearest in total
+ * particles, within a factor of 4, where the autotuner found one at least
+ * 2 % faster than the built-in plan (hermite_amd/csrc/hh_plan_table.hpp),
+ * otherwise from the built-in 64, 256, 8192, 256 (split waves 8192,
+ * 16384), which measured within 1.3 % of the best plan found for 4th order
+ * N = 1024, a 256-cluster ensemble of 550k particles, 6th order N = 2^16 and
+ * 2^20 and 8th order N = 2^18. */
 int hhr_set_plan(hhr_ctx *ctx, int jpar_max, int smax, int task_target, int jpar_target);
 /* i-parallel tasks of clusters with at most wj4_max (wj2_max) active
  * particles split each j-tile over 4 (2) waves of the workgroup, so a task
