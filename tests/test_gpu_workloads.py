@@ -209,3 +209,27 @@ def test_c5_recipe_energy_vs_oracle_at_small_n():
         assert abs(st["num_bstep"] / o.num_bstep - 1) <= 0.02
         assert st["dtmin_hits"] == 0
         assert de <= 2 * de_o + 1e-13
+
+
+@pytest.mark.parametrize("name,tend", [("c2", 1.0), ("c3", 1 / 32)])
+def test_split_integration_energy_within_twice_fp64(name, tend):
+    """The north star's integration bound on the benchmark workloads: the
+    split-precision resident run reaches the end time with |dE/E| within
+    twice the FP64 run's, in the same number of block steps to 2 % (the
+    6th-order N = 2^16 run to t = 1; the 8th-order N = 2^18 run to
+    t = 1/32, a quarter of its span)."""
+    w = W.workload(name)
+    c = w.clusters[0]
+    P, e0 = init_on_gpu(w, c)
+    out = {}
+    for variant in ("fp64", "split"):
+        r = H.Resident(w.order, variant, [(P, c.eps ** 2, w.eta, w.dtmax, w.dtmin, 0.0)])
+        try:
+            r.run(tend)
+            st = r.stats()
+            out[variant] = (st["num_bstep"], abs((r.energy() - e0) / e0))
+        finally:
+            r.close()
+        print(f"{name} {variant} t={tend:g}: {out[variant][0]} block steps, |dE/E| {out[variant][1]:.3e}", flush=True)
+    assert abs(out["split"][0] / out["fp64"][0] - 1) <= 0.02
+    assert out["split"][1] <= 2 * out["fp64"][1]
