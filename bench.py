@@ -301,9 +301,9 @@ def cpu_baseline(w, states, cpu_tend, budget_s=20.0):
         ref = oracle.RefAVX(w.order)
     except (ImportError, FileNotFoundError):
         return None
-    info = _cpu_info()
-    threads = _cpu_threads(info)
-    cpu = info["model"]
+    cpuinfo = _cpu_info()
+    threads = _cpu_threads(cpuinfo)
+    cpu = cpuinfo["model"]
     # thread placement of the reference's OpenMP runs (child processes)
     os.environ["OMP_PLACES"] = "cores"
     os.environ["OMP_PROC_BIND"] = "close"
@@ -354,7 +354,7 @@ def cpu_baseline(w, states, cpu_tend, budget_s=20.0):
                   + f" ({secs:.1f} s), {cpu}")
     out = {"value": inter / secs, "unit": "interactions/s", "cores": used, "kind": "reference",
            "sample": sample, "gflops": inter / secs * H.FLOPS[w.order] * 1e-9,
-           "per_core": inter / secs / max(used, 1), "cpu": info,
+           "per_core": inter / secs / max(used, 1), "cpu": cpuinfo,
            "placement": "OMP_PLACES=cores OMP_PROC_BIND=close"}
     if len(states) == 1 and len(states[0][0]) <= 65536 and len(states[0][0]) % 4 == 0:
         # the reference's "Init N square" evaluation: all N on all N, once
