@@ -97,8 +97,8 @@ def integrate(w, variant, states, tend, timing=False, graph=False, chunks=1, wal
     """Resident integration t = 0 -> tend, in `chunks` equal time spans
     (each hhr_run call ends with a stream synchronisation; a progress line
     per chunk).  wall_s counts the hhr_run calls only.  With t_energy
-    (0 < t_energy < tend, a chunk boundary after rounding up) |dE/E| is also
-    recorded at that time, between two hhr_run calls.  With wall_limit (s)
+    (0 < t_energy <= tend, a multiple of dt_max) |dE/E| is also recorded at
+    that time, between two hhr_run calls (None when the run ends before).  With wall_limit (s)
     the run stops after the first chunk that ends past it; with max_bsteps a
     chunk stops after that many block steps."""
     clus = [(P0, eps2, w.eta, w.dtmax, w.dtmin, 0.0) for P0, eps2, _, _ in states]
@@ -140,7 +140,7 @@ def integrate(w, variant, states, tend, timing=False, graph=False, chunks=1, wal
             out["dE_E"] = float(de.max())
             if len(de) > 1:
                 out["dE_E_median"] = float(np.median(de))
-            if t_energy is not None and t_energy >= tend:
+            if t_energy is not None and t_energy == tend:
                 de_at = de
         if de_at is not None:
             out["dE_E_at_t_energy"] = float(de_at.max())
