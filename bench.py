@@ -411,7 +411,7 @@ def split_roofline(w, avg_ms, achieved, launches):
     interaction, issued at 78.6/2 and 157.3/2 T per second, the blended peak
     is flops / (n64 / 39.3 + n32 / 78.65) TF/s."""
     mix = opmix_from_profile(w.order)
-    out = {"bound": "valu", "kernel": f"r_force_kernel<{w.order},SPLIT>", "achieved": achieved,
+    out = {"bound": "valu", "kernel": force_kernel_name(w, "SPLIT"), "achieved": achieved,
            "unit": "TFLOP/s", "avg_launch_ms": avg_ms, "launches": launches,
            "peak_fp64": PEAK_FP64_TFLOPS, "frac_fp64": achieved / PEAK_FP64_TFLOPS,
            "peak_fp32": PEAK_FP32_TFLOPS, "frac_fp32": achieved / PEAK_FP32_TFLOPS, "opmix": mix}
@@ -421,8 +421,24 @@ def split_roofline(w, avg_ms, achieved, launches):
     return out
 
 
+FUSED_MAX = 4096  # single clusters up to this size take the two-launch block step
+
+
+def force_kernel_name(w, mode):
+    """The force kernel the GPU-resident block steps launch: the two-launch
+    step's r_fused_force_kernel<ORDER,MODE,JPL> for one cluster of at most
+    FUSED_MAX particles (JPL: j per lane, 2 up to 1024 particles and at 8th
+    order, else 4), r_force_kernel<ORDER,MODE> otherwise."""
+    if len(w.clusters) == 1 and w.n_total <= FUSED_MAX:
+        jpl = 2 if (w.n_total <= 1024 or w.order == 8) else 4
+        return f"r_fused_force_kernel<{w.order},{mode},{jpl}>"
+    return f"r_force_kernel<{w.order},{mode}>"
+
+
 def workload_text(w, tend, graph):
     kind = "hipGraph replay" if graph else "direct launches on one stream"
+    if len(w.clusters) == 1 and w.n_total <= FUSED_MAX:
+        kind += ", two launches per block step"
     if w.name == "c4":
         return (f"4th-order Hermite ensemble of {len(w.clusters)} Plummer clusters, N_k 256..8192 "
                 f"(sum {w.n_total}), eps=4/N_k, eta=0.1, dt_max=1/16, t=0->{tend:g}, plain FP64 kernel, "
@@ -567,7 +583,7 @@ def main():
         "dtmin_hits": f["dtmin_hits"],
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic_from_profile(w),
-                     "kernel": f"r_force_kernel<{w.order},FP64>", "flops_per_interaction": H.FLOPS[w.order],
+                     "kernel": force_kernel_name(w, "FP64"), "flops_per_interaction": H.FLOPS[w.order],
                      "avg_launch_ms": avg_ms, "launches": ft["force_launches"],
                      "force_share_of_timed_wall": ft["force_ms"] * 1e-3 / ft["wall_s"],
                      "by_active_count": by_active_count(ft["log"], n_j)},

@@ -735,6 +735,10 @@ int hh_last_geometry(const hh_ctx *c, int *kernel, int *jsplit, int *workgroups)
 struct hhr_ctx {
     int order = 0, variant = 0, device = 0, nclu = 0, npad = 0, ntiles = 0, rw = 0;
     int force_grid = 0, reduce_grid = 0;
+    // two-launch block steps for one cluster of at most fused_max
+    // particles (FP64 and split): about fused_grid force workgroups,
+    // fused_jpl j per lane (0: by size)
+    int fused_max = R_FUSED_MAX, fused_grid = 512, fused_jpl = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double *rp = nullptr, *jpr = nullptr, *ipr = nullptr, *tn = nullptr, *part = nullptr, *pot = nullptr;
@@ -826,8 +830,57 @@ static hipError_t r_launch_step(hhr_ctx *c, int s)
     return hipGetLastError();
 }
 
+// One block step of a small single cluster in two launches (list + force,
+// correct + bookkeeping); the events bracket the first as the force launch.
+// Its partials (one per j-slice and slot) fit in the partial buffer, which
+// holds at least NT * R_TASK_MAX force records.
+static_assert((size_t)R_FUSED_SLICES * R_FUSED_MAX <= (size_t)NT * R_TASK_MAX, "fused partials");
+template <int ORDER, int MODE, int JPL>
+static hipError_t r_launch_fused_jpl(hhr_ctx *c, int s)
+{
+    const RArgs a = r_args(c);
+    const int n = c->n[0], off = c->off[0];
+    const int S = (n + JPL * NT - 1) / (JPL * NT);   // j-slices
+    const int wps = std::max(1, c->fused_grid / S);  // workgroups per slice
+    hipError_t e;
+    if ((e = r_mark(c, 4 * s)) != hipSuccess || (e = r_mark(c, 4 * s + 1)) != hipSuccess) return e;
+    hipLaunchKernelGGL((r_fused_force_kernel<ORDER, MODE, JPL>), dim3(wps * S), dim3(NT), 0, c->stream, a, n, off, S, wps);
+    if ((e = r_mark(c, 4 * s + 2)) != hipSuccess) return e;
+    hipLaunchKernelGGL((r_fused_correct_kernel<ORDER>), dim3((n + NT - 1) / NT), dim3(NT), 0, c->stream, a, n, off, S);
+    if ((e = r_mark(c, 4 * s + 3)) != hipSuccess) return e;
+    return hipGetLastError();
+}
+
+// j per lane of the force launch; 0 picks by size: two up to 1024
+// particles and at 8th order (four would leave one wave per SIMD), four
+// above (measured on MI355X at N = 1024, 2048 and 4096)
+template <int ORDER, int MODE>
+static hipError_t r_launch_fused_step(hhr_ctx *c, int s)
+{
+    const int n = c->n[0];
+    const int jpl = c->fused_jpl ? c->fused_jpl : (n <= 1024 || ORDER == 8) ? 2 : 4;
+    if (jpl == 1) return r_launch_fused_jpl<ORDER, MODE, 1>(c, s);
+    if (jpl == 2) return r_launch_fused_jpl<ORDER, MODE, 2>(c, s);
+    return r_launch_fused_jpl<ORDER, MODE, 4>(c, s);
+}
+
+static bool r_fused(const hhr_ctx *c)
+{
+    return c->nclu == 1 && c->variant != HH_EXACT && c->n[0] <= c->fused_max;
+}
+
 static hipError_t r_step(hhr_ctx *c, int s)
 {
+    if (r_fused(c)) {
+        switch (c->order * 4 + c->variant) {
+        case 16 + HH_FP64: return r_launch_fused_step<4, FP64>(c, s);
+        case 16 + HH_SPLIT: return r_launch_fused_step<4, SPLIT>(c, s);
+        case 24 + HH_FP64: return r_launch_fused_step<6, FP64>(c, s);
+        case 24 + HH_SPLIT: return r_launch_fused_step<6, SPLIT>(c, s);
+        case 32 + HH_FP64: return r_launch_fused_step<8, FP64>(c, s);
+        default: return r_launch_fused_step<8, SPLIT>(c, s);
+        }
+    }
     switch (c->order * 4 + c->variant) {
     case 16 + HH_FP64: return r_launch_step<4, FP64>(c, s);
     case 16 + HH_SPLIT: return r_launch_step<4, SPLIT>(c, s);
@@ -1191,6 +1244,24 @@ int hhr_set_split_waves(hhr_ctx *c, int wj4_max, int wj2_max)
     const int v[2] = {wj4_max, wj2_max};
     RCHK(c, hipMemcpy((char *)c->glob + offsetof(RGlob, plan) + offsetof(RPlan, wj4_max), v, sizeof(v),
                       hipMemcpyHostToDevice));
+    return HH_OK;
+}
+
+int hhr_set_fused(hhr_ctx *c, int max_n, int grid, int jpl)
+{
+    if (!c || max_n < 0 || max_n > R_FUSED_MAX || grid < 0 || grid > 4096 || !(jpl == 0 || jpl == 1 || jpl == 2 || jpl == 4))
+        return rfail(c, HH_E_ARG, "fused-step parameters out of range");
+    RCHK(c, hipSetDevice(c->device));
+    RCHK(c, hipStreamSynchronize(c->stream));
+    c->fused_max = max_n;
+    c->fused_grid = grid ? grid : 512;
+    c->fused_jpl = jpl;
+    // the captured batch holds the launches of the old choice
+    if (c->gexec) RCHK(c, hipGraphExecDestroy(c->gexec));
+    if (c->graph) RCHK(c, hipGraphDestroy(c->graph));
+    c->gexec = nullptr;
+    c->graph = nullptr;
+    c->graph_steps = 0;
     return HH_OK;
 }
 

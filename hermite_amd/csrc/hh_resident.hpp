@@ -987,4 +987,294 @@ __global__ __launch_bounds__(NT) void r_reduce_correct_kernel(RArgs a)
     }
 }
 
+// ------------------------------------------------------ two-launch step
+// A block step of one small cluster (n <= R_FUSED_MAX) in two launches
+// instead of three, for the FP64 and split variants.  Short block steps
+// are bound by the latency of dependent memory round trips, not by
+// arithmetic, so both kernels issue every load that does not depend on the
+// block time in their first round trip.
+//
+// r_fused_force_kernel: the n slots are cut into S = ceil(n / (JPL NT))
+// j-slices of JPL NT slots, JPL per lane; workgroup b works on slice b % S
+// for the groups of R_GI active particles b / S, b / S + wps, ... (wps
+// workgroups per slice).  Each workgroup loads its lanes' j records and all n block
+// times at once, builds the active list in LDS itself (a redundant scan:
+// cheaper than a launch boundary at these sizes), predicts its i records
+// into LDS, and for each group sums the lanes' interactions in a fixed
+// order into the slice's partial force at the particle's slot.  Nothing in
+// this launch writes the records, so every workgroup predicts from the
+// corrected state of the previous step.  Workgroup 0 writes the active
+// count.
+//
+// r_fused_correct_kernel: one thread per slot.  Active slots sum their S
+// partials in slice order and apply the corrector; every slot's new (or
+// unchanged) block time goes into the next step's minimum.  The last
+// workgroup to finish (an atomic count, as in the compaction) does the
+// step's bookkeeping of r_plan_body for one cluster.  No workgroup reads
+// what another one of the same launch writes.
+//
+// A one-launch step, whose last force workgroup corrects after a
+// device-scope release/acquire around its count, measured slower on
+// MI355X (19.3 us against 9.8 + 6.5 us at N = 1024, 4th order): every
+// workgroup's release writes back its XCD's L2, and 512 of those cost
+// more than a launch boundary.
+constexpr int R_FUSED_MAX = 4096;
+constexpr int R_FUSED_TV = R_FUSED_MAX / NT;  // block times per thread in the scan
+constexpr int R_FUSED_SLICES = R_FUSED_MAX / NT;  // at most, with one j per lane
+
+// Block time, step limit and collapse test of one cluster, from the slot
+// minima of both parities loaded together with the step counter.
+struct RFusedStep {
+    double tmin, dtlim;
+    int p;
+    bool live, collapsed, stepping;
+};
+
+__device__ __forceinline__ RFusedStep r_fused_step(const RArgs &a, const RClu &c)
+{
+    const int lane = threadIdx.x & 63;
+    const unsigned long long none = 0x7ff0000000000000ULL;
+    const unsigned long long b0 = lane < R_SLOTS ? c.tmin_slot[0][lane] : none;
+    const unsigned long long b1 = lane < R_SLOTS ? c.tmin_slot[1][lane] : none;
+    RFusedStep f;
+    f.p = a.glob->step & 1;
+    f.tmin = r_wave_min(__longlong_as_double((long long)(f.p ? b1 : b0)));
+    f.live = f.tmin <= c.tend;
+    f.dtlim = f.live ? r_dtlim(f.tmin, c.dtmax) : 0.0;
+    f.collapsed = f.live && (!(f.tmin > c.tsys) || f.dtlim <= 0.0);
+    f.stepping = f.live && !f.collapsed;
+    return f;
+}
+
+// The step's bookkeeping of r_plan_body for one cluster, by one thread
+// after every other workgroup of the corrector launch is done: counters,
+// the cluster's time and step limit, the cleared parity buffers of this
+// step, and the step counter.
+__device__ void r_fused_bookkeep(const RArgs &a, RClu &c, const RFusedStep &st, int nact, int n, int S)
+{
+    const int p = st.p;
+    a.glob->cdone[p] = 0;
+    c.acnt[p] = 0;
+    c.mode = 0;
+    c.ntask = 0;
+    c.tnext = st.tmin;
+#pragma unroll
+    for (int q = 0; q < R_SLOTS; q++) c.tmin_slot[p][q] = r_bits(r_inf());
+    c.nact = st.stepping ? nact : 0;
+    if (st.collapsed) {
+        c.tend = c.tsys;
+        atomicExch(&a.glob->error, 1);
+    }
+    const int ngrp = (nact + R_GI - 1) / R_GI;
+    const bool live_n = st.stepping && nact > 0;
+    if (live_n) {
+        c.dtlim = st.dtlim;
+        c.num_step += nact;
+        c.num_bstep += 1;
+        c.tsys = st.tmin;
+        c.mode = 2;
+        c.ngrp = ngrp;
+        c.nslice = S;
+        c.jchunk = (n + S - 1) / S;
+        c.ntask = ngrp * S;
+    }
+    c.task0 = 0;
+    c.act0 = 0;
+    c.pofs = 0;
+    a.glob->total_tasks = live_n ? ngrp * S : 0;
+    a.glob->nactive = live_n ? 1 : 0;
+    a.glob->total_active = live_n ? nact : 0;
+    if (a.steplog) a.steplog[a.glob->step % a.steplog_cap] = live_n ? nact : 0;
+    a.glob->step += 1;
+}
+
+template <int ORDER, int MODE, int JPL>
+__global__ __launch_bounds__(NT) void r_fused_force_kernel(RArgs a, int n, int off, int S, int wps)
+{
+    static_assert(MODE != EXACT, "EXACT sums j in index order: three-launch step");
+    constexpr int W = jp_words(ORDER), Q = pr_words(ORDER);
+    constexpr int NC = f_words(ORDER);
+    constexpr int V = R_GI * NC;
+    constexpr int NW = NT / 64;
+    constexpr int SW = MODE == SPLIT ? split_words(ORDER) : Q;  // words of the body's records
+    __shared__ int s_list[R_FUSED_MAX];
+    __shared__ int s_wcnt[R_FUSED_TV][NW];
+    __shared__ double s_pi[SW][NT];
+    __shared__ double red[2][NW][V];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int slice = blockIdx.x % S, g0 = blockIdx.x / S;  // the grid is wps * S workgroups
+    RClu &c = a.clu[0];
+    const double eps2 = c.eps2;
+
+    // first round trip: step and block-time minima, all block times, this
+    // lane's j records
+    double tv[R_FUSED_TV];
+#pragma unroll
+    for (int k = 0; k < R_FUSED_TV; k++) {
+        const int i = k * NT + tid;
+        tv[k] = i < n ? a.tn[off + i] : r_inf();
+    }
+    double rj[JPL][W];
+#pragma unroll
+    for (int u = 0; u < JPL; u++) {
+        const int j = (slice * JPL + u) * NT + tid;
+#pragma unroll
+        for (int w = 0; w < W; w++) rj[u][w] = j < n ? a.rp[(size_t)w * a.npad + off + j] : 0.0;
+    }
+    const RFusedStep st = r_fused_step(a, c);
+
+    // active list in slot order
+    unsigned long long amask[R_FUSED_TV];
+#pragma unroll
+    for (int k = 0; k < R_FUSED_TV; k++) {
+        amask[k] = __ballot(st.stepping && tv[k] == st.tmin);
+        if (lane == 0) s_wcnt[k][wave] = __popcll(amask[k]);
+    }
+    __syncthreads();
+    int nact = 0;
+    const unsigned long long below = lane == 0 ? 0ULL : (~0ULL) >> (64 - lane);
+#pragma unroll
+    for (int k = 0; k < R_FUSED_TV; k++) {
+        int before = nact;
+        for (int w = 0; w < NW; w++) {
+            if (w < wave) before += s_wcnt[k][w];
+            nact += s_wcnt[k][w];
+        }
+        if ((amask[k] >> lane) & 1ULL) s_list[before + __popcll(amask[k] & below)] = k * NT + tid;
+    }
+    if (blockIdx.x == 0 && tid == 0) a.glob->total_active = nact;
+
+    // this lane's j, predicted to the block time
+    double qj[JPL][SW];
+#pragma unroll
+    for (int u = 0; u < JPL; u++) {
+        double q[Q];
+        predict_words<ORDER, false>(rj[u], st.tmin, q);
+        if constexpr (MODE == SPLIT) {
+            to_split<ORDER>(q, qj[u]);
+        } else {
+#pragma unroll
+            for (int w = 0; w < Q; w++) qj[u][w] = q[w];
+        }
+    }
+    const int nj = min(JPL, max(0, (n - slice * JPL * NT - tid + NT - 1) / NT));  // this lane's j
+    const int ngrp = (nact + R_GI - 1) / R_GI;
+    const int mygrp = g0 < ngrp && g0 < wps ? (ngrp - g0 + wps - 1) / wps : 0;
+    int rb = 0;  // red buffer
+    for (int k0 = 0; k0 < mygrp; k0 += NT / R_GI) {
+        const int kn = min(NT / R_GI, mygrp - k0);
+        __syncthreads();  // s_list written; s_pi free
+        // predicted i records of this round's groups, one particle per thread
+        if (tid < kn * R_GI) {
+            const int lp = min((g0 + (k0 + tid / R_GI) * wps) * R_GI + tid % R_GI, nact - 1);
+            const int ip = off + s_list[lp];
+            double r[W], q[Q];
+#pragma unroll
+            for (int w = 0; w < W; w++) r[w] = a.rp[(size_t)w * a.npad + ip];
+            predict_words<ORDER, false>(r, st.tmin, q);
+            if constexpr (MODE == SPLIT) {
+                double t[SW];
+                to_split<ORDER>(q, t);
+#pragma unroll
+                for (int w = 0; w < SW; w++) s_pi[w][tid] = t[w];
+            } else {
+#pragma unroll
+                for (int w = 0; w < Q; w++) s_pi[w][tid] = q[w];
+            }
+        }
+        __syncthreads();
+        for (int kk = 0; kk < kn; kk++) {
+            const int grp = g0 + (k0 + kk) * wps;
+            double A[R_GI][NC];
+#pragma unroll
+            for (int g = 0; g < R_GI; g++)
+#pragma unroll
+                for (int q = 0; q < NC; q++) A[g][q] = 0.0;
+#pragma unroll
+            for (int g = 0; g < R_GI; g++) {
+                double pi[SW];
+#pragma unroll
+                for (int w = 0; w < SW; w++) pi[w] = s_pi[w][kk * R_GI + g];
+                float Af[n_f32(ORDER)];
+#pragma unroll
+                for (int q = 0; q < n_f32(ORDER); q++) Af[q] = 0.0f;
+#pragma unroll
+                for (int u = 0; u < JPL; u++) {
+                    if (u < nj) {
+                        if constexpr (MODE == FP64) body_fp64<ORDER, false>(pi, qj[u], eps2, false, 0, A[g]);
+                        else body_split<ORDER, false>(pi, qj[u], eps2, false, A[g], Af);
+                    }
+                }
+                if constexpr (MODE == SPLIT) split_fold<ORDER>(A[g], Af);
+            }
+            // workgroup sums in a fixed order: shuffle tree per wave, waves
+            // 0..3; the two red buffers alternate, one barrier per group
+#pragma unroll
+            for (int g = 0; g < R_GI; g++)
+#pragma unroll
+                for (int q = 0; q < NC; q++) {
+                    const double x = wave_sum(A[g][q]);
+                    if (lane == 0) red[rb][wave][g * NC + q] = x;
+                }
+            __syncthreads();
+            if (tid < V) {
+                double tot = 0.0;
+#pragma unroll
+                for (int w = 0; w < NW; w++) tot += red[rb][w][tid];
+                const int g = tid / NC, q = tid - g * NC;
+                const int lp = grp * R_GI + g;
+                if (lp < nact) a.part[((size_t)slice * R_FUSED_MAX + s_list[lp]) * NC + q] = tot;
+            }
+            rb ^= 1;
+        }
+    }
+}
+
+template <int ORDER>
+__global__ __launch_bounds__(NT) void r_fused_correct_kernel(RArgs a, int n, int off, int S)
+{
+    constexpr int NC = f_words(ORDER);
+    constexpr int NW = NT / 64;
+    __shared__ double s_wmin[NW];
+    __shared__ int s_last;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    RClu &c = a.clu[0];
+    const int i = blockIdx.x * NT + tid;
+    const double told = i < n ? a.tn[off + i] : r_inf();
+    const int nact = a.glob->total_active;
+    const RFusedStep st = r_fused_step(a, c);
+    double tnew = told;
+    if (st.stepping && told == st.tmin) {
+        RCluR cr{};
+        cr.etapow = c.etapow;
+        cr.dtlim = st.dtlim;
+        cr.dtmin = c.dtmin;
+        double f[NC];
+#pragma unroll
+        for (int q = 0; q < NC; q++) f[q] = 0.0;
+        for (int sl = 0; sl < S; sl++)
+#pragma unroll
+            for (int q = 0; q < NC; q++) f[q] += a.part[((size_t)sl * R_FUSED_MAX + i) * NC + q];
+        tnew = r_finish<ORDER>(a, cr, off + i, f);
+    }
+    // minimum of this workgroup's new block times into the next step's
+    // minimum (every slot of the cluster is covered by one thread)
+    double m = r_wave_min(tnew);
+    if (lane == 0) s_wmin[wave] = m;
+    __syncthreads();
+    // the last workgroup does the step's bookkeeping (r_plan_body for one
+    // cluster); every wave of a workgroup has read the step counter, the
+    // block-time minima and the active count before the workgroup counts
+    // itself done
+    if (tid == 0) {
+#pragma unroll
+        for (int w = 1; w < NW; w++) m = fmin(m, s_wmin[w]);
+        if (m < r_inf()) atomicMin(&c.tmin_slot[st.p ^ 1][blockIdx.x % R_SLOTS], r_bits(m));
+        __builtin_amdgcn_s_waitcnt(0);
+        s_last = atomicAdd(&a.glob->cdone[st.p], 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (s_last && tid == 0) r_fused_bookkeep(a, c, st, nact, n, S);
+}
+
 }  // namespace hh
