@@ -425,13 +425,17 @@ FUSED_MAX = 4096  # single clusters up to this size take the two-launch block st
 
 
 def force_kernel_name(w, mode):
-    """The force kernel the GPU-resident block steps launch: the two-launch
-    step's r_fused_force_kernel<ORDER,MODE,JPL> for one cluster of at most
-    FUSED_MAX particles (JPL: j per lane, 2 up to 1024 particles and at 8th
-    order, else 4), r_force_kernel<ORDER,MODE> otherwise."""
-    if len(w.clusters) == 1 and w.n_total <= FUSED_MAX:
-        jpl = 2 if (w.n_total <= 1024 or w.order == 8) else 4
-        return f"r_fused_force_kernel<{w.order},{mode},{jpl}>"
+    """The force kernel the GPU-resident block steps launch: for one cluster
+    of at most FUSED_MAX particles the two-launch step's
+    r_fused_islice_kernel<ORDER,MODE,JPL> (up to 1024 particles, 2048 at
+    4th order) or r_fused_force_kernel<ORDER,MODE,JPL> (JPL: j per lane, 2
+    up to 1024 particles and at 8th order, else 4); r_force_kernel<ORDER,MODE>
+    otherwise."""
+    n = w.n_total
+    if len(w.clusters) == 1 and n <= FUSED_MAX:
+        jpl = 2 if (n <= 1024 or w.order == 8) else 4
+        kern = "r_fused_islice_kernel" if (n <= 1024 or (w.order == 4 and n <= 2048)) else "r_fused_force_kernel"
+        return f"{kern}<{w.order},{mode},{jpl}>"
     return f"r_force_kernel<{w.order},{mode}>"
 
 

@@ -61,7 +61,7 @@ def hip_lib():
         L.hh_predict_all.argtypes = [vp, C.c_double]
         L.hh_force_first_nact.argtypes = [vp, C.c_int, C.c_double, _dp, _dp]
         L.hh_force_first_nact_static.argtypes = [vp, C.c_int, C.c_double, _dp, _dp]
-        L.hhr_set_fused.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+        L.hhr_set_fused.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
         L.hh_static_scratch_doubles.restype = C.c_longlong
         L.hh_potential.argtypes = [vp, C.c_double, _dp]
         L.hh_last_error.argtypes = [vp]
@@ -513,11 +513,13 @@ class Resident:
         j-tile over 4 (2) waves (see hhr_set_split_waves)."""
         self._check(hip_lib().hhr_set_split_waves(self.h, wj4_max, wj2_max), "hhr_set_split_waves")
 
-    def set_fused(self, max_n=4096, grid=0, jpl=0):
+    def set_fused(self, max_n=4096, grid=0, jpl=0, islice_max=0):
         """Two-launch block steps for a single cluster of at most max_n
-        particles (0 disables; grid and jpl 0: defaults; see
+        particles (0 disables; grid, jpl and islice_max 0: defaults,
+        islice_max -1: the active-list force kernel only; see
         hhr_set_fused)."""
-        self._check(hip_lib().hhr_set_fused(self.h, int(max_n), int(grid), int(jpl)), "hhr_set_fused")
+        self._check(hip_lib().hhr_set_fused(self.h, int(max_n), int(grid), int(jpl), int(islice_max)),
+                    "hhr_set_fused")
 
     def set_timing(self, on=True):
         """Bracket every force launch with HIP events (resets the totals);

@@ -737,8 +737,9 @@ struct hhr_ctx {
     int force_grid = 0, reduce_grid = 0;
     // two-launch block steps for one cluster of at most fused_max
     // particles (FP64 and split): about fused_grid force workgroups,
-    // fused_jpl j per lane (0: by size)
-    int fused_max = R_FUSED_MAX, fused_grid = 512, fused_jpl = 0;
+    // fused_jpl j per lane (0: by size), the i-slice force kernel up to
+    // fused_islice_max particles (0: by size, -1: never)
+    int fused_max = R_FUSED_MAX, fused_grid = 512, fused_jpl = 0, fused_islice_max = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double *rp = nullptr, *jpr = nullptr, *ipr = nullptr, *tn = nullptr, *part = nullptr, *pot = nullptr;
@@ -844,7 +845,18 @@ static hipError_t r_launch_fused_jpl(hhr_ctx *c, int s)
     const int wps = std::max(1, c->fused_grid / S);  // workgroups per slice
     hipError_t e;
     if ((e = r_mark(c, 4 * s)) != hipSuccess || (e = r_mark(c, 4 * s + 1)) != hipSuccess) return e;
-    hipLaunchKernelGGL((r_fused_force_kernel<ORDER, MODE, JPL>), dim3(wps * S), dim3(NT), 0, c->stream, a, n, off, S, wps);
+    // i-slice force kernel (no dependent round trip) up to 1024 particles,
+    // 2048 at 4th order; above, the active list built in every workgroup
+    // balances crowded steps better (measured on MI355X, orders 4/6/8)
+    const bool islice = c->fused_islice_max ? n <= c->fused_islice_max : (n <= 1024 || (ORDER == 4 && n <= 2048));
+    if (islice) {
+        const int Si = (n + NT - 1) / NT;
+        const int wpc = std::max(1, c->fused_grid / (Si * S));  // workgroups per (i-slice, j-slice)
+        hipLaunchKernelGGL((r_fused_islice_kernel<ORDER, MODE, JPL>), dim3(wpc * Si * S), dim3(NT), 0, c->stream, a, n, off, Si, S,
+                           wpc);
+    } else {
+        hipLaunchKernelGGL((r_fused_force_kernel<ORDER, MODE, JPL>), dim3(wps * S), dim3(NT), 0, c->stream, a, n, off, S, wps);
+    }
     if ((e = r_mark(c, 4 * s + 2)) != hipSuccess) return e;
     hipLaunchKernelGGL((r_fused_correct_kernel<ORDER>), dim3((n + NT - 1) / NT), dim3(NT), 0, c->stream, a, n, off, S);
     if ((e = r_mark(c, 4 * s + 3)) != hipSuccess) return e;
@@ -1247,15 +1259,17 @@ int hhr_set_split_waves(hhr_ctx *c, int wj4_max, int wj2_max)
     return HH_OK;
 }
 
-int hhr_set_fused(hhr_ctx *c, int max_n, int grid, int jpl)
+int hhr_set_fused(hhr_ctx *c, int max_n, int grid, int jpl, int islice_max)
 {
-    if (!c || max_n < 0 || max_n > R_FUSED_MAX || grid < 0 || grid > 4096 || !(jpl == 0 || jpl == 1 || jpl == 2 || jpl == 4))
+    if (!c || max_n < 0 || max_n > R_FUSED_MAX || grid < 0 || grid > 4096 || !(jpl == 0 || jpl == 1 || jpl == 2 || jpl == 4) ||
+        islice_max < -1 || islice_max > R_FUSED_MAX)
         return rfail(c, HH_E_ARG, "fused-step parameters out of range");
     RCHK(c, hipSetDevice(c->device));
     RCHK(c, hipStreamSynchronize(c->stream));
     c->fused_max = max_n;
     c->fused_grid = grid ? grid : 512;
     c->fused_jpl = jpl;
+    c->fused_islice_max = islice_max;
     // the captured batch holds the launches of the old choice
     if (c->gexec) RCHK(c, hipGraphExecDestroy(c->gexec));
     if (c->graph) RCHK(c, hipGraphDestroy(c->graph));

@@ -140,9 +140,20 @@ struct RPart {  // one particle's words, loaded from / stored to the SoA record
     __device__ void set3(int k, const D3 &v) const { w(k) = v.x; w(k + 1) = v.y; w(k + 2) = v.z; }
 };
 
-// f: acc jrk [snp [crk]] of this particle.  Returns the new dt.
-template <int ORDER>
-__device__ void r_correct(const RPart &p, const double *f, double etapow, double dtlim)
+// One particle's words held in registers (loaded and stored by the caller),
+// with the accessors of RPart.
+template <int RW>
+struct RRec {
+    mutable double v[RW];
+    __device__ double &w(int k) const { return v[k]; }
+    __device__ D3 v3(int k) const { return D3(v[k], v[k + 1], v[k + 2]); }
+    __device__ void set3(int k, const D3 &x) const { v[k] = x.x; v[k + 1] = x.y; v[k + 2] = x.z; }
+};
+
+// f: acc jrk [snp [crk]] of this particle.  Sets the new dt.  P: RPart
+// (the SoA record in memory) or RRec (the words in registers).
+template <int ORDER, class P>
+__device__ void r_correct(const P &p, const double *f, double etapow, double dtlim)
 {
     const double dt = p.w(R_DT);
     const unsigned long long b = (unsigned long long)__double_as_longlong(dt) - (1ULL << 52);
@@ -1003,15 +1014,18 @@ __global__ __launch_bounds__(NT) void r_reduce_correct_kernel(RArgs a)
 // into LDS, and for each group sums the lanes' interactions in a fixed
 // order into the slice's partial force at the particle's slot.  Nothing in
 // this launch writes the records, so every workgroup predicts from the
-// corrected state of the previous step.  Workgroup 0 writes the active
-// count.
+// corrected state of the previous step.  (r_fused_islice_kernel below is
+// the variant without the scan for the smallest clusters.)
 //
-// r_fused_correct_kernel: one thread per slot.  Active slots sum their S
-// partials in slice order and apply the corrector; every slot's new (or
-// unchanged) block time goes into the next step's minimum.  The last
+// r_fused_correct_kernel: one thread per slot, whose block time, record
+// and slice partials are all loaded in one round trip.  Active slots sum
+// their S partials in slice order and apply the corrector; every slot's
+// new (or unchanged) block time goes into the next step's minimum and each
+// workgroup adds its active count to the step's counter.  The last
 // workgroup to finish (an atomic count, as in the compaction) does the
 // step's bookkeeping of r_plan_body for one cluster.  No workgroup reads
-// what another one of the same launch writes.
+// what another one of the same launch writes, except the counters, which
+// it reads as atomics.
 //
 // A one-launch step, whose last force workgroup corrects after a
 // device-scope release/acquire around its count, measured slower on
@@ -1142,7 +1156,6 @@ __global__ __launch_bounds__(NT) void r_fused_force_kernel(RArgs a, int n, int o
         }
         if ((amask[k] >> lane) & 1ULL) s_list[before + __popcll(amask[k] & below)] = k * NT + tid;
     }
-    if (blockIdx.x == 0 && tid == 0) a.glob->total_active = nact;
 
     // this lane's j, predicted to the block time
     double qj[JPL][SW];
@@ -1233,48 +1246,216 @@ __global__ __launch_bounds__(NT) void r_fused_force_kernel(RArgs a, int n, int o
 template <int ORDER>
 __global__ __launch_bounds__(NT) void r_fused_correct_kernel(RArgs a, int n, int off, int S)
 {
-    constexpr int NC = f_words(ORDER);
+    constexpr int NC = f_words(ORDER), RW = r_words(ORDER);
     constexpr int NW = NT / 64;
     __shared__ double s_wmin[NW];
+    __shared__ int s_wcnt[NW];
     __shared__ int s_last;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     RClu &c = a.clu[0];
     const int i = blockIdx.x * NT + tid;
-    const double told = i < n ? a.tn[off + i] : r_inf();
-    const int nact = a.glob->total_active;
-    const RFusedStep st = r_fused_step(a, c);
-    double tnew = told;
-    if (st.stepping && told == st.tmin) {
-        RCluR cr{};
-        cr.etapow = c.etapow;
-        cr.dtlim = st.dtlim;
-        cr.dtmin = c.dtmin;
-        double f[NC];
+    const bool in = i < n;
+    // one round trip: the slot's block time, record and slice partials are
+    // loaded before the block time is known (inactive slots discard them)
+    const double told = in ? a.tn[off + i] : r_inf();
+    RRec<RW> rr;
 #pragma unroll
-        for (int q = 0; q < NC; q++) f[q] = 0.0;
+    for (int k = 0; k < RW; k++) rr.v[k] = in ? a.rp[(size_t)k * a.npad + off + i] : 0.0;
+    double f[NC];
+#pragma unroll
+    for (int q = 0; q < NC; q++) f[q] = 0.0;
+    if (in)
         for (int sl = 0; sl < S; sl++)
 #pragma unroll
             for (int q = 0; q < NC; q++) f[q] += a.part[((size_t)sl * R_FUSED_MAX + i) * NC + q];
-        tnew = r_finish<ORDER>(a, cr, off + i, f);
+    const double etapow = c.etapow, dtmin = c.dtmin;
+    const RFusedStep st = r_fused_step(a, c);
+    const bool act = st.stepping && told == st.tmin;
+    double tnew = told;
+    if (act) {  // r_finish on the words in registers
+        r_correct<ORDER>(rr, f, etapow, st.dtlim);
+        double dt = rr.v[R_DT];
+        if (dtmin > 0.0 && dt < dtmin) {
+            dt = dtmin;
+            rr.v[R_DT] = dt;
+            atomicAdd((unsigned long long *)&c.dtmin_hits, 1ULL);
+        }
+        tnew = rr.v[7] + dt;
+        a.tn[off + i] = tnew;
+#pragma unroll
+        for (int k = 0; k < RW; k++)
+            if (k != 3 && k != R_ID) a.rp[(size_t)k * a.npad + off + i] = rr.v[k];
+        if (!isfinite(rr.v[0] + rr.v[1] + rr.v[2] + rr.v[4] + rr.v[5] + rr.v[6])) atomicExch(&a.glob->error, 2);
     }
-    // minimum of this workgroup's new block times into the next step's
-    // minimum (every slot of the cluster is covered by one thread)
+    // this workgroup's active count, and the minimum of its new block times
+    // into the next step's minimum (every slot of the cluster is covered by
+    // one thread)
     double m = r_wave_min(tnew);
-    if (lane == 0) s_wmin[wave] = m;
+    const int cnt = __popcll(__ballot(act));
+    if (lane == 0) {
+        s_wmin[wave] = m;
+        s_wcnt[wave] = cnt;
+    }
     __syncthreads();
     // the last workgroup does the step's bookkeeping (r_plan_body for one
-    // cluster); every wave of a workgroup has read the step counter, the
-    // block-time minima and the active count before the workgroup counts
-    // itself done
+    // cluster); every wave of a workgroup has read the step counter and the
+    // block-time minima, and the workgroup's count and minimum have been
+    // performed, before the workgroup counts itself done
     if (tid == 0) {
+        int tot = s_wcnt[0];
 #pragma unroll
-        for (int w = 1; w < NW; w++) m = fmin(m, s_wmin[w]);
+        for (int w = 1; w < NW; w++) {
+            m = fmin(m, s_wmin[w]);
+            tot += s_wcnt[w];
+        }
         if (m < r_inf()) atomicMin(&c.tmin_slot[st.p ^ 1][blockIdx.x % R_SLOTS], r_bits(m));
+        if (tot) atomicAdd(&c.acnt[st.p], tot);
         __builtin_amdgcn_s_waitcnt(0);
         s_last = atomicAdd(&a.glob->cdone[st.p], 1u) == gridDim.x - 1;
     }
     __syncthreads();
-    if (s_last && tid == 0) r_fused_bookkeep(a, c, st, nact, n, S);
+    if (s_last && tid == 0) {
+        const int nact = __hip_atomic_load(&c.acnt[st.p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        r_fused_bookkeep(a, c, st, nact, n, S);
+    }
+}
+
+// Force launch of the two-launch step for clusters of at most 1024 or so
+// particles, with no dependent round trip: the slots are cut into Si
+// i-slices of NT slots (one per lane) and Sj j-slices of JPL NT slots;
+// workgroup b takes the combination (b mod Si, (b / Si) mod Sj) and the
+// groups b / (Si Sj), b / (Si Sj) + wpc, ... of the active particles of
+// its i-slice.  Every load (its i-slice's block times and records, its
+// lanes' j records) is issued before the block time is known; the active
+// lanes of the i-slice put their predicted records in LDS in slot order.
+// A group's i are taken from one i-slice, so active particles crowded
+// into few slots leave workgroups of the other i-slices idle; the kernel
+// of the global active list above balances those steps.
+template <int ORDER, int MODE, int JPL>
+__global__ __launch_bounds__(NT) void r_fused_islice_kernel(RArgs a, int n, int off, int Si, int Sj, int wpc)
+{
+    static_assert(MODE != EXACT, "EXACT sums j in index order: three-launch step");
+    constexpr int W = jp_words(ORDER), Q = pr_words(ORDER);
+    constexpr int NC = f_words(ORDER);
+    constexpr int V = R_GI * NC;
+    constexpr int NW = NT / 64;
+    constexpr int SW = MODE == SPLIT ? split_words(ORDER) : Q;
+    __shared__ double s_pi[SW][NT];
+    __shared__ int s_slot[NT];
+    __shared__ int s_wcnt[NW];
+    __shared__ double red[2][NW][V];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int combo = blockIdx.x % (Si * Sj), g0 = blockIdx.x / (Si * Sj);
+    const int si = combo % Si, sj = combo / Si;
+    RClu &c = a.clu[0];
+    const double eps2 = c.eps2;
+
+    // the only round trip
+    const int islot = si * NT + tid;
+    const bool iin = islot < n;
+    const double ti = iin ? a.tn[off + islot] : r_inf();
+    double ri[W];
+#pragma unroll
+    for (int w = 0; w < W; w++) ri[w] = iin ? a.rp[(size_t)w * a.npad + off + islot] : 0.0;
+    double rj[JPL][W];
+#pragma unroll
+    for (int u = 0; u < JPL; u++) {
+        const int j = (sj * JPL + u) * NT + tid;
+#pragma unroll
+        for (int w = 0; w < W; w++) rj[u][w] = j < n ? a.rp[(size_t)w * a.npad + off + j] : 0.0;
+    }
+    const RFusedStep st = r_fused_step(a, c);
+
+    // the i-slice's active particles, predicted, in slot order
+    const bool act = st.stepping && ti == st.tmin;
+    const unsigned long long mask = __ballot(act);
+    if (lane == 0) s_wcnt[wave] = __popcll(mask);
+    if (act) {
+        double q[Q];
+        predict_words<ORDER, false>(ri, st.tmin, q);
+        if constexpr (MODE == SPLIT) {
+            double t[SW];
+            to_split<ORDER>(q, t);
+#pragma unroll
+            for (int w = 0; w < SW; w++) ri[w] = t[w];
+        } else {
+#pragma unroll
+            for (int w = 0; w < Q; w++) ri[w] = q[w];
+        }
+    }
+    double qj[JPL][SW];
+#pragma unroll
+    for (int u = 0; u < JPL; u++) {
+        double q[Q];
+        predict_words<ORDER, false>(rj[u], st.tmin, q);
+        if constexpr (MODE == SPLIT) {
+            to_split<ORDER>(q, qj[u]);
+        } else {
+#pragma unroll
+            for (int w = 0; w < Q; w++) qj[u][w] = q[w];
+        }
+    }
+    const int nj = min(JPL, max(0, (n - sj * JPL * NT - tid + NT - 1) / NT));  // this lane's j
+    __syncthreads();
+    int pos = 0, na = 0;
+#pragma unroll
+    for (int w = 0; w < NW; w++) {
+        if (w < wave) pos += s_wcnt[w];
+        na += s_wcnt[w];
+    }
+    if (act) {
+        pos += __popcll(mask & (lane == 0 ? 0ULL : (~0ULL) >> (64 - lane)));
+#pragma unroll
+        for (int w = 0; w < SW; w++) s_pi[w][pos] = ri[w];
+        s_slot[pos] = islot;
+    }
+    __syncthreads();
+    const int ngrp = (na + R_GI - 1) / R_GI;
+    int rb = 0;  // red buffer
+    for (int grp = g0; grp < ngrp; grp += wpc) {
+        double A[R_GI][NC];
+#pragma unroll
+        for (int g = 0; g < R_GI; g++)
+#pragma unroll
+            for (int q = 0; q < NC; q++) A[g][q] = 0.0;
+#pragma unroll
+        for (int g = 0; g < R_GI; g++) {
+            const int lp = min(grp * R_GI + g, na - 1);
+            double pi[SW];
+#pragma unroll
+            for (int w = 0; w < SW; w++) pi[w] = s_pi[w][lp];
+            float Af[n_f32(ORDER)];
+#pragma unroll
+            for (int q = 0; q < n_f32(ORDER); q++) Af[q] = 0.0f;
+#pragma unroll
+            for (int u = 0; u < JPL; u++) {
+                if (u < nj) {
+                    if constexpr (MODE == FP64) body_fp64<ORDER, false>(pi, qj[u], eps2, false, 0, A[g]);
+                    else body_split<ORDER, false>(pi, qj[u], eps2, false, A[g], Af);
+                }
+            }
+            if constexpr (MODE == SPLIT) split_fold<ORDER>(A[g], Af);
+        }
+        // workgroup sums in a fixed order: shuffle tree per wave, waves
+        // 0..3; the two red buffers alternate, one barrier per group
+#pragma unroll
+        for (int g = 0; g < R_GI; g++)
+#pragma unroll
+            for (int q = 0; q < NC; q++) {
+                const double x = wave_sum(A[g][q]);
+                if (lane == 0) red[rb][wave][g * NC + q] = x;
+            }
+        __syncthreads();
+        if (tid < V) {
+            double tot = 0.0;
+#pragma unroll
+            for (int w = 0; w < NW; w++) tot += red[rb][w][tid];
+            const int g = tid / NC, q = tid - g * NC;
+            const int lp = grp * R_GI + g;
+            if (lp < na) a.part[((size_t)sj * R_FUSED_MAX + s_slot[lp]) * NC + q] = tot;
+        }
+        rb ^= 1;
+    }
 }
 
 }  // namespace hh

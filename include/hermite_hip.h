@@ -252,13 +252,16 @@ int hhr_set_split_waves(hhr_ctx *ctx, int wj4_max, int wj2_max);
  * integrated with HH_FP64 or HH_SPLIT runs each block step as two launches
  * instead of three.  The force launch has about `grid` workgroups (0:
  * 512), cut evenly over j-slices of 256 jpl slots (jpl: 1, 2 or 4 j per
- * lane; 0 picks by size); every workgroup builds the active list itself
- * and predicts its i and j on the fly.  The corrector launch, one thread
- * per slot, sums the slices in order and its last workgroup does the
- * step's bookkeeping.  HH_EXACT keeps the three-launch step, whose j
- * summation order is the reference's.  A particle's force differs from the
- * three-launch step's in the j summation order only. */
-int hhr_set_fused(hhr_ctx *ctx, int max_n, int grid, int jpl);
+ * lane; 0 picks by size) and predicts its i and j on the fly; up to
+ * islice_max particles (0: 1024, at 4th order 2048; -1: never) each
+ * workgroup takes its i from one slice of 256 slots and issues every load
+ * in one round trip, above that every workgroup builds the whole active
+ * list itself.  The corrector launch, one thread per slot, sums the slices
+ * in order and its last workgroup does the step's bookkeeping.  HH_EXACT
+ * keeps the three-launch step, whose j summation order is the
+ * reference's.  A particle's force differs from the three-launch step's
+ * in the j summation order only. */
+int hhr_set_fused(hhr_ctx *ctx, int max_n, int grid, int jpl, int islice_max);
 
 /* Force-kernel timing (no reference counterpart): when on, hhr_run launches
  * its steps directly on the context's stream (no graph; the kernels are the

@@ -225,7 +225,7 @@ def test_resident_fp64_reduce_paths(n):
     res = {}
     for variant in ("exact", "fp64"):
         r = H.Resident(order, variant, [(P0, eps2, eta, 1 / 16, 0.0, 0.0)])
-        r.set_fused(0)  # the reduce kernel's paths, not the single-launch step
+        r.set_fused(0)  # the reduce kernel's paths, not the two-launch step
         r.set_timing(True)
         r.run(1 / 32)
         na, _, _ = r.step_log()
@@ -291,7 +291,7 @@ def test_resident_edge_sizes(n):
     ens.run(1 / 8, use_graph=True, batch=4)
     for k, P in enumerate(clus):
         one = H.Resident(6, "fp64", [(P, eps2, eta, 1 / 16, 0.0, 0.0)])
-        one.set_fused(0)  # the ensemble's launches, not the single-launch step
+        one.set_fused(0)  # the ensemble's launches, not the two-launch step
         one.run(1 / 8, use_graph=True, batch=4)
         assert one.stats() == ens.stats(k)
         assert np.array_equal(one.particles(), ens.particles(k))
@@ -311,7 +311,7 @@ def test_plan_and_split_wave_settings_are_independent():
     out = []
     for how in ("split-then-plan", "plan-then-split", "default"):
         r = H.Resident(order, "fp64", [(P0, eps2, 0.2, 1 / 16, 0.0, 0.0)])
-        r.set_fused(0)  # the planned launches, not the single-launch step
+        r.set_fused(0)  # the planned launches, not the two-launch step
         if how == "split-then-plan":
             r.set_split_waves(0, 0)
             r.set_plan()
@@ -348,38 +348,40 @@ def test_large_ensemble_bitwise_vs_oracle_inplace():
 @pytest.mark.parametrize("order", ORDERS)
 @pytest.mark.parametrize("variant", ["fp64", "split"])
 def test_fused_step_equals_planned_step(order, variant):
-    """The two-launch block step of small clusters (hhr_set_fused) against
-    the three-launch step (compaction, planned force, reduce + correct) on
-    the same cluster: the same block steps and particle steps, positions
-    and energies equal to rounding (only the j summation order differs), at
-    N = 257 (ragged), 1024, 2048 and 4096 (the largest fused size; one, two
-    and four j per lane, and the size's default); with graph replay and
-    direct launches, and a step-limited run that resumes."""
+    """The two-launch block step of small clusters (hhr_set_fused), with
+    either force kernel (i-slices, or the active list built in every
+    workgroup), against the three-launch step (compaction, planned force,
+    reduce + correct) on the same cluster: the same block steps and
+    particle steps, positions and energies equal to rounding (only the j
+    summation order differs), at N = 257 (ragged), 1024, 2048 and 4096 (the
+    largest fused size; one, two and four j per lane, and the size's
+    default); with graph replay and direct launches, and a step-limited run
+    that resumes."""
     eps2, eta, eta_s = (1 / 64) ** 2, {4: 0.1, 6: 0.3, 8: 0.5}[order], {4: 0.01, 6: 0.1, 8: 0.1}[order]
     for n, jpl in ((257, 1), (1024, 2), (2048, 0), (4096, 4)):
         o = initialised(order, n, 81, eps2, eta, eta_s, 1 / 16)
         P0, e0 = o.ptcl(), o.init_energy
         out = {}
-        for launches in (2, 3):
+        for kind in ("islice", "list", "planned"):
             r = H.Resident(order, variant, [(P0, eps2, eta, 1 / 16, 0.0, 0.0)])
             try:
-                if launches == 3:
+                if kind == "planned":
                     r.set_fused(0)
                 else:
-                    r.set_fused(4096, 0, jpl)
+                    r.set_fused(4096, 0, jpl, 4096 if kind == "islice" else -1)
                     r.set_step_limit(7)
                     with pytest.raises(H.StepLimitReached):
                         r.run(1 / 16, use_graph=False, batch=4)
                     r.set_step_limit(0)
                 r.run(1 / 16, use_graph=True, batch=8)
-                out[launches] = (r.stats(), r.particles(), r.energy())
+                out[kind] = (r.stats(), r.particles(), r.energy())
             finally:
                 r.close()
-        sp, pp, ep = out[3]
-        for launches in (2,):
-            sf, pf, ef = out[launches]
+        sp, pp, ep = out["planned"]
+        for kind in ("islice", "list"):
+            sf, pf, ef = out[kind]
             dx = np.abs(pf[:, 4:7] - pp[:, 4:7]).max()
-            print(f"{launches}-launch order={order} {variant} n={n}: block steps {sf['num_bstep']} / {sp['num_bstep']}, "
+            print(f"two-launch ({kind}) order={order} {variant} n={n}: block steps {sf['num_bstep']} / {sp['num_bstep']}, "
                   f"max |dx| {dx:.2e}, dE {abs((ef - ep) / e0):.2e}", flush=True)
             assert sf["tsys"] == sp["tsys"] == 1 / 16
             assert abs(sf["num_step"] / sp["num_step"] - 1) < 0.01
