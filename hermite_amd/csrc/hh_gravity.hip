@@ -820,9 +820,12 @@ static hipError_t r_launch_step(hhr_ctx *c, int s)
     const RArgs a = r_args(c);
     hipError_t e = r_mark(c, 4 * s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((r_compact_kernel<ORDER, MODE == EXACT>), dim3(c->ntiles), dim3(NT), 0, c->stream, a);
-    if (c->nclu > 1)  // one cluster: planned by the compaction's last workgroup
+    if (c->nclu == 1) {  // one cluster: planned by the compaction's last workgroup
+        hipLaunchKernelGGL((r_compact_kernel<ORDER, MODE == EXACT, true>), dim3(c->ntiles), dim3(NT), 0, c->stream, a);
+    } else {
+        hipLaunchKernelGGL((r_compact_kernel<ORDER, MODE == EXACT, false>), dim3(c->ntiles), dim3(NT), 0, c->stream, a);
         hipLaunchKernelGGL((r_plan_kernel<ORDER>), dim3(1), dim3(NT), 0, c->stream, a);
+    }
     if ((e = r_mark(c, 4 * s + 1)) != hipSuccess) return e;
     hipLaunchKernelGGL((r_force_kernel<ORDER, MODE>), dim3(c->force_grid), dim3(NT), 0, c->stream, a);
     if ((e = r_mark(c, 4 * s + 2)) != hipSuccess) return e;

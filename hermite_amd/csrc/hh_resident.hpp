@@ -75,7 +75,7 @@ struct RClu {
     int wj;              // mode 1: waves sharing each tile's j (1, 2 or 4)
     int ngrp, nslice, tps, jchunk;
     int task0, ntask, act0, ipad;
-    long long pofs;      // partials part[pofs + (s*NC + c)*ipad + ii]
+    long long pofs;      // partials part[pofs + (s*ipad + ii)*NC + c]: a slice's words of one particle together
 };
 
 struct RGlob {
@@ -84,7 +84,10 @@ struct RGlob {
     int total_active;
     int step;      // block steps planned; step & 1 selects the current buffers
     RPlan plan;
-    unsigned int cdone[2];  // compaction workgroups done, parity buffered (one cluster)
+    unsigned int cdone[2];  // fused corrector workgroups done, parity buffered (one cluster)
+    // one cluster's compaction: workgroups done (high 32 bits) and active
+    // particles appended (low 32 bits) in one word, parity buffered
+    unsigned long long cpack[2];
 };
 
 struct RArgs {
@@ -139,6 +142,14 @@ struct RPart {  // one particle's words, loaded from / stored to the SoA record
     __device__ D3 v3(int k) const { return D3(w(k), w(k + 1), w(k + 2)); }
     __device__ void set3(int k, const D3 &v) const { w(k) = v.x; w(k + 1) = v.y; w(k + 2) = v.z; }
 };
+
+// Record words the corrector of an order reads or writes (r_correct): 4th
+// order pos vel tlast acc jrk snp crk and dt; 6th order also d4a d5a; 8th
+// order every word but the mass and the id, which it never touches.
+__host__ __device__ constexpr bool r_used(int o, int k)
+{
+    return k != 3 && k != R_ID && (o == 8 || k == R_DT || k < (o == 4 ? 20 : 26));
+}
 
 // One particle's words held in registers (loaded and stored by the caller),
 // with the accessors of RPart.
@@ -265,23 +276,27 @@ __device__ __forceinline__ RCluR r_clu_rfields(const RClu &c)
     return f;
 }
 
-// Correct particle i with force f, clamp dt to dtmin, refresh tn[i].
+// Correct particle i, whose record the caller has loaded into rr, with
+// force f; clamp dt to dtmin, store the record and refresh tn[i].
 template <int ORDER>
-__device__ double r_finish(const RArgs &a, const RCluR &c, int i, const double *f)
+__device__ double r_finish(const RArgs &a, const RCluR &c, int i, RRec<r_words(ORDER)> &rr, const double *f)
 {
-    const RPart p{a.rp, a.npad, i};
-    r_correct<ORDER>(p, f, c.etapow, c.dtlim);
-    double dt = p.w(R_DT);
+    constexpr int RW = r_words(ORDER);
+    r_correct<ORDER>(rr, f, c.etapow, c.dtlim);
+    double dt = rr.v[R_DT];
     if (c.dtmin > 0.0 && dt < c.dtmin) {
         dt = c.dtmin;
-        p.w(R_DT) = dt;
+        rr.v[R_DT] = dt;
         atomicAdd((unsigned long long *)&a.clu[a.tile_clu[i / TJ]].dtmin_hits, 1ULL);
     }
-    const double tn = p.w(7) + dt;
+    const double tn = rr.v[7] + dt;
     a.tn[i] = tn;
+#pragma unroll
+    for (int k = 0; k < RW; k++)
+        if (r_used(ORDER, k)) a.rp[(size_t)k * a.npad + i] = rr.v[k];
     // a non-finite state (a step that ran away on noisy derivatives) stops
     // the run instead of integrating garbage
-    if (!isfinite(p.w(0) + p.w(1) + p.w(2) + p.w(4) + p.w(5) + p.w(6))) atomicExch(&a.glob->error, 2);
+    if (!isfinite(rr.v[0] + rr.v[1] + rr.v[2] + rr.v[4] + rr.v[5] + rr.v[6])) atomicExch(&a.glob->error, 2);
     return tn;
 }
 
@@ -307,30 +322,83 @@ __device__ __forceinline__ double r_tmin_wave(const RClu &c, int p)
 template <int ORDER>
 __device__ void r_plan_body(const RArgs &a, bool atomic_counts);
 
+// Fields of a single cluster that its plan reads, loaded by every workgroup
+// of the compaction in its first round trip, so that the last workgroup
+// plans from registers instead of reading them after its count.
+struct RPlanIn {
+    double tsys, dtmax;
+    long long num_step, num_bstep;
+    int ntiles, exact;
+    RPlan plan;
+};
+
+__device__ __forceinline__ RPlanIn r_plan_in(const RArgs &a, const RClu &c)
+{
+    RPlanIn f;
+    f.tsys = c.tsys;
+    f.dtmax = c.dtmax;
+    f.num_step = c.num_step;
+    f.num_bstep = c.num_bstep;
+    f.ntiles = c.ntiles;
+    f.exact = a.glob->exact;
+    f.plan = a.glob->plan;
+    return f;
+}
+
+template <int ORDER>
+__device__ void r_plan_one(const RArgs &a, RClu &c, const RPlanIn &in, int step, int nact, double tmin);
+
 // One workgroup per 256-slot tile; a tile belongs to one cluster.  Times
 // are positive, and positive doubles order like their bit patterns, so the
-// minima are integer atomicMin on the bits.  With one cluster the last
-// workgroup to finish also plans the step (r_plan_body), which saves the
-// plan kernel's launch and its dependent reads.
-template <int ORDER, bool EX>
+// minima are integer atomicMin on the bits.
+//
+// One cluster: every load of the kernel (both parities of the block-time
+// minimum, the step counter, the slot's block time and record, the fields
+// the plan reads) is issued in one round trip, and one 64-bit atomic both
+// reserves the tile's list positions and counts the workgroup done; the
+// last workgroup to finish plans the step from registers (r_plan_one).
+// That saves the plan kernel's launch and every dependent read but the
+// atomic.  Ensembles append with one atomic per tile and plan in
+// r_plan_kernel.
+template <int ORDER, bool EX, bool ONE>
 __global__ __launch_bounds__(NT) void r_compact_kernel(RArgs a)
 {
     constexpr int W = jp_words(ORDER), Q = pr_words(ORDER);
     constexpr int NW = NT / 64;
     __shared__ double wmin[NW];
     __shared__ int wcnt[NW];
-    __shared__ int s_base;
+    __shared__ int s_base, s_last, s_nact;
     const int tile = blockIdx.x, tid = threadIdx.x;
-    const int k = a.tile_clu[tile];
+    const int lane = tid & 63, wave = tid >> 6;
+    const int k = ONE ? 0 : a.tile_clu[tile];
     RClu &c = a.clu[k];
-    const int p = a.glob->step & 1;
-    const double tmin = r_tmin_wave(c, p);
-    const bool live = tmin <= c.tend;
     const int i = tile * TJ + tid;
+    const int step = a.glob->step;
+    const unsigned long long none = 0x7ff0000000000000ULL;
+    const unsigned long long b0 = lane < R_SLOTS ? c.tmin_slot[0][lane] : none;
+    const unsigned long long b1 = lane < R_SLOTS ? c.tmin_slot[1][lane] : none;
+    const double tend = c.tend;
     const double t = a.tn[i];
+    RPlanIn pin{};
+    if constexpr (ONE) pin = r_plan_in(a, c);
+    // One cluster: the record is read before the block time is known
+    // (slot i is inside the store, i < npad), but only after the loads
+    // above have been issued: loads complete in issue order for the wait
+    // counter, so the schedule's first step waits for those few words alone
+    // while the records stream in (at N = 2^18 they are 54 MB per step).
+    // Ensembles read only the records of clusters still live: most are
+    // done during the last block steps of a run.
+    __builtin_amdgcn_sched_barrier(0);
+    double r[W];
+    if constexpr (ONE) {
+#pragma unroll
+        for (int w = 0; w < W; w++) r[w] = a.rp[(size_t)w * a.npad + i];
+    }
+    const int p = step & 1;
+    const double tmin = r_wave_min(__longlong_as_double((long long)(p ? b1 : b0)));
+    const bool live = tmin <= tend;
     const bool act = live && t == tmin;
     const unsigned long long mask = __ballot(act);
-    const int lane = tid & 63, wave = tid >> 6;
     // minimum block time of the slots this step leaves alone
     double m = r_wave_min(act ? r_inf() : t);
     if (lane == 0) {
@@ -346,14 +414,24 @@ __global__ __launch_bounds__(NT) void r_compact_kernel(RArgs a)
             m = fmin(m, wmin[w]);
             tot += wcnt[w];
         }
-        s_base = tot > 0 ? atomicAdd(&c.acnt[p], tot) : 0;
         if (m < r_inf()) atomicMin(&c.tmin_slot[p ^ 1][tile % R_SLOTS], r_bits(m));
+        if constexpr (ONE) {
+            const unsigned long long old = atomicAdd(&a.glob->cpack[p], (1ULL << 32) | (unsigned long long)tot);
+            s_base = (int)(old & 0xffffffffULL);
+            s_nact = s_base + tot;
+            s_last = (unsigned)(old >> 32) == gridDim.x - 1;
+        } else {
+            s_base = tot > 0 ? atomicAdd(&c.acnt[p], tot) : 0;
+        }
     }
     double q[Q];
-    if (live) {
-        double r[W];
+    if constexpr (!ONE) {
+        if (live) {
 #pragma unroll
-        for (int w = 0; w < W; w++) r[w] = a.rp[(size_t)w * a.npad + i];
+            for (int w = 0; w < W; w++) r[w] = a.rp[(size_t)w * a.npad + i];
+        }
+    }
+    if (live) {
         predict_words<ORDER, EX>(r, tmin, q);
 #pragma unroll
         for (int w = 0; w < Q; w++) a.jpr[(size_t)w * a.npad + i] = q[w];
@@ -369,19 +447,10 @@ __global__ __launch_bounds__(NT) void r_compact_kernel(RArgs a)
 #pragma unroll
         for (int w = 0; w < Q; w++) a.ipr[(size_t)w * a.npad + pos] = q[w];
     }
-    if (a.nclu == 1) {
-        __shared__ int s_last;
-        if (tid == 0) {
-            // this tile's append has been performed (its atomic returned)
-            // before the count of finished workgroups goes up
-            __builtin_amdgcn_s_waitcnt(0);
-            s_last = atomicAdd(&a.glob->cdone[p], 1u) == gridDim.x - 1;
-        }
-        __syncthreads();
-        if (s_last) {
-            if (tid == 0) a.glob->cdone[p] = 0;  // next used two steps on
-            r_plan_body<ORDER>(a, true);
-        }
+    // the last workgroup's count returned the total: every other workgroup
+    // has appended; the plan reads nothing else they wrote
+    if constexpr (ONE) {
+        if (s_last) r_plan_one<ORDER>(a, c, pin, step, s_nact, tmin);
     }
 }
 
@@ -425,6 +494,41 @@ __device__ T r_block_exscan(T v, T *s_buf, T *total)
     __syncthreads();
     *total = all;
     return before + x - v;
+}
+
+// Task geometry of one cluster's block step from its active count and its
+// share of the step's task budget: i-parallel groups (mode 1) or
+// j-parallel groups (mode 2), j-slices, tiles per slice or j per chunk.
+struct RGeom {
+    int mode, wj, ngrp, tps, jchunk, nslice, ipad;
+};
+
+__device__ __forceinline__ RGeom r_geometry(const RPlan &pl, bool exact, int nact, int ntiles, double share)
+{
+    RGeom g{};
+    if (exact || nact > pl.jpar_max) {
+        const int nil = 64 * r_ni(exact ? EXACT : FP64);  // i per wave of a mode-1 task
+        const int budget = max(1, (int)ceil(pl.task_target * share));
+        g.mode = 1;
+        // EXACT sums each tile's j in index order on one wave
+        g.wj = exact ? 1 : nact <= pl.wj4_max ? 4 : nact <= pl.wj2_max ? 2 : 1;
+        const int gs = nil * (NT / 64 / g.wj);
+        g.ngrp = (nact + gs - 1) / gs;
+        const int ns = exact ? 1 : min(min(ntiles, pl.smax), max(1, (budget + g.ngrp - 1) / g.ngrp));
+        g.tps = (ntiles + ns - 1) / ns;
+        g.nslice = (ntiles + g.tps - 1) / g.tps;
+        g.ipad = g.ngrp * gs;
+    } else {
+        const int budget = max(1, (int)ceil(pl.jpar_target * share));
+        g.mode = 2;
+        g.ngrp = (nact + R_GI - 1) / R_GI;
+        const int ns = min(ntiles, max(1, (budget + g.ngrp - 1) / g.ngrp));
+        const int tpc = (ntiles + ns - 1) / ns;  // tiles per chunk
+        g.jchunk = tpc * TJ;
+        g.nslice = (ntiles + tpc - 1) / tpc;
+        g.ipad = g.ngrp * R_GI;
+    }
+    return g;
 }
 
 // One workgroup; thread t owns the clusters [t*cpt, (t+1)*cpt).  Geometry
@@ -474,7 +578,6 @@ __device__ void r_plan_body(const RArgs &a, bool atomic_counts)
 #pragma unroll
     for (int q = 0; q < NW; q++) total += s_wsum[q];
 
-    const int nil = 64 * r_ni(exact ? EXACT : FP64);  // i per wave of a mode-1 task
     int my_tasks = 0, my_act = 0, my_live = 0;
     long long my_part = 0;
     for (int k = k0; k < k1; k++) {
@@ -493,28 +596,15 @@ __device__ void r_plan_body(const RArgs &a, bool atomic_counts)
         c.num_bstep += 1;
         c.tsys = tmin;
         const double share = ((double)c.nact * c.n) / total;
-        if (exact || c.nact > pl.jpar_max) {
-            const int budget = max(1, (int)ceil(pl.task_target * share));
-            c.mode = 1;
-            // EXACT sums each tile's j in index order on one wave
-            c.wj = exact ? 1 : c.nact <= pl.wj4_max ? 4 : c.nact <= pl.wj2_max ? 2 : 1;
-            const int gs = nil * (NT / 64 / c.wj);
-            c.ngrp = (c.nact + gs - 1) / gs;
-            const int ns = exact ? 1 : min(min(c.ntiles, pl.smax), max(1, (budget + c.ngrp - 1) / c.ngrp));
-            c.tps = (c.ntiles + ns - 1) / ns;
-            c.nslice = (c.ntiles + c.tps - 1) / c.tps;
-            c.ipad = c.ngrp * gs;
-        } else {
-            const int budget = max(1, (int)ceil(pl.jpar_target * share));
-            c.mode = 2;
-            c.ngrp = (c.nact + R_GI - 1) / R_GI;
-            const int ns = min(c.ntiles, max(1, (budget + c.ngrp - 1) / c.ngrp));
-            const int tpc = (c.ntiles + ns - 1) / ns;  // tiles per chunk
-            c.jchunk = tpc * TJ;
-            c.nslice = (c.ntiles + tpc - 1) / tpc;
-            c.ipad = c.ngrp * R_GI;
-        }
-        c.ntask = c.ngrp * c.nslice;
+        const RGeom g = r_geometry(pl, exact, c.nact, c.ntiles, share);
+        c.mode = g.mode;
+        c.wj = g.wj;
+        c.ngrp = g.ngrp;
+        c.tps = g.tps;
+        c.jchunk = g.jchunk;
+        c.nslice = g.nslice;
+        c.ipad = g.ipad;
+        c.ntask = g.ngrp * g.nslice;
         my_tasks += c.ntask;
         my_act += c.nact;
         my_live += 1;
@@ -550,6 +640,55 @@ template <int ORDER>
 __global__ __launch_bounds__(NT) void r_plan_kernel(RArgs a)
 {
     r_plan_body<ORDER>(a, false);
+}
+
+// The plan of one cluster's block step (r_plan_body for nclu = 1, with the
+// whole task budget), by the last workgroup of the compaction: the active
+// count and the block time come from the compaction itself, the cluster's
+// fields from its first round trip.  Clears this step's parity buffers for
+// their next use and advances the step counter.
+template <int ORDER>
+__device__ void r_plan_one(const RArgs &a, RClu &c, const RPlanIn &in, int step, int nact, double tmin)
+{
+    const int p = step & 1, tid = threadIdx.x;
+    if (tid < R_SLOTS) c.tmin_slot[p][tid] = r_bits(r_inf());
+    if (tid != 0) return;
+    a.glob->cpack[p] = 0ULL;  // next used two steps on
+    c.tnext = tmin;
+    int ntask = 0, act = 0;
+    RGeom g{};
+    if (nact > 0) {
+        const double dtlim = r_dtlim(tmin, in.dtmax);
+        if (!(tmin > in.tsys) || dtlim <= 0.0) {  // collapsed step: stop the cluster
+            c.tend = in.tsys;
+            atomicExch(&a.glob->error, 1);
+        } else {
+            c.dtlim = dtlim;
+            c.num_step = in.num_step + nact;
+            c.num_bstep = in.num_bstep + 1;
+            c.tsys = tmin;
+            g = r_geometry(in.plan, in.exact != 0, nact, in.ntiles, 1.0);
+            c.wj = g.wj;
+            c.ngrp = g.ngrp;
+            c.tps = g.tps;
+            c.jchunk = g.jchunk;
+            c.nslice = g.nslice;
+            c.ipad = g.ipad;
+            ntask = g.ngrp * g.nslice;
+            act = nact;
+        }
+    }
+    c.nact = act;
+    c.mode = g.mode;
+    c.ntask = ntask;
+    c.task0 = 0;
+    c.act0 = 0;
+    c.pofs = 0;
+    a.glob->total_tasks = ntask;
+    a.glob->nactive = act > 0 ? 1 : 0;
+    a.glob->total_active = act;
+    if (a.steplog) a.steplog[step % a.steplog_cap] = act;
+    a.glob->step = step + 1;
 }
 
 // The fields of a cluster record that a force task reads.
@@ -780,7 +919,7 @@ __device__ __forceinline__ void r_force_body(const RArgs &a)
                 for (int k = 0; k < NI; k++)
                     if (ii[k] < c.nact) {
 #pragma unroll
-                        for (int q = 0; q < NC; q++) part[((size_t)slice * NC + q) * c.ipad + ii[k]] = A[k][q];
+                        for (int q = 0; q < NC; q++) part[((size_t)slice * c.ipad + ii[k]) * NC + q] = A[k][q];
                     }
             }
         } else if constexpr (MODE != EXACT) {  // EXACT always plans mode 1
@@ -854,7 +993,7 @@ __device__ __forceinline__ void r_force_body(const RArgs &a)
             }
             if (tid < V) {
                 const int g = tid / NC, q = tid - g * NC;
-                a.part[c.pofs + ((size_t)slice * NC + q) * c.ipad + grp * R_GI + g] = tot;
+                a.part[c.pofs + ((size_t)slice * c.ipad + grp * R_GI + g) * NC + q] = tot;
             }
         }
     }
@@ -899,9 +1038,9 @@ __device__ __forceinline__ double r_lane_tree(double x)
 
 template <int ORDER, int RLX>
 __device__ __forceinline__ void r_reduce_body(const RArgs &a, const int *s_key, bool keys_lds, int nclu, int total,
-                                              int nb, const RCluR &c0, double &tacc, int &kacc)
+                                              int nb, const RCluR &c0, int ip_first, double &tacc, int &kacc)
 {
-    constexpr int NC = f_words(ORDER);
+    constexpr int NC = f_words(ORDER), RW = r_words(ORDER);
     constexpr int PB = NT / RLX;  // particles per workgroup pass
     const int l = threadIdx.x % RLX;
     for (int base = blockIdx.x * PB; base < total; base += gridDim.x * PB) {
@@ -910,9 +1049,15 @@ __device__ __forceinline__ void r_reduce_body(const RArgs &a, const int *s_key, 
         const int k = !live || nclu == 1 ? 0 : keys_lds ? r_find_lds(s_key, nclu, pos) : r_find_active(a.clu, nclu, pos);
         const RCluR c = nclu == 1 ? c0 : r_clu_rfields(a.clu[k]);
         const int ii = pos - c.act0;
-        // the particle's index, read before the slice sums so that its
-        // record can be read as soon as they are done
-        const int ip = live && l == 0 ? a.ilist[c.off + ii] : 0;
+        // the particle's index (one cluster: read at kernel start for the
+        // first pass) and its record, loaded before the slice sums so that
+        // both are in flight together
+        const int ip = live && l == 0 ? (nclu == 1 && base == (int)blockIdx.x * PB ? ip_first : a.ilist[c.off + ii]) : 0;
+        RRec<RW> rr;
+        if (live && l == 0) {
+#pragma unroll
+            for (int w = 0; w < RW; w++) rr.v[w] = r_used(ORDER, w) ? a.rp[(size_t)w * a.npad + ip] : 0.0;
+        }
         double S[NC];
 #pragma unroll
         for (int q = 0; q < NC; q++) S[q] = 0.0;
@@ -921,13 +1066,13 @@ __device__ __forceinline__ void r_reduce_body(const RArgs &a, const int *s_key, 
 #pragma unroll 4
             for (int sl = l; sl < c.nslice; sl += RLX) {
 #pragma unroll
-                for (int q = 0; q < NC; q++) S[q] += part[((size_t)sl * NC + q) * c.ipad + ii];
+                for (int q = 0; q < NC; q++) S[q] += part[((size_t)sl * c.ipad + ii) * NC + q];
             }
         }
 #pragma unroll
         for (int q = 0; q < NC; q++) S[q] = r_lane_tree<RLX>(S[q]);
         if (live && l == 0) {
-            const double tn = r_finish<ORDER>(a, c, ip, S);
+            const double tn = r_finish<ORDER>(a, c, ip, rr, S);
             if (kacc != k && kacc >= 0) atomicMin(&a.clu[kacc].tmin_slot[nb][blockIdx.x % R_SLOTS], r_bits(tacc));
             if (kacc != k) tacc = r_inf();
             kacc = k;
@@ -941,9 +1086,16 @@ __global__ __launch_bounds__(NT) void r_reduce_correct_kernel(RArgs a)
 {
     __shared__ int s_key[R_CLU_LDS];
     const int nclu = a.nclu;
-    // a single cluster's fields are read together with the step's counts
+    // a single cluster's fields, and this thread's first active-list entry
+    // (for 64 and for RL lanes per particle; its list starts at position
+    // 0), are read together with the step's counts
     RCluR c0{};
-    if (nclu == 1) c0 = r_clu_rfields(a.clu[0]);
+    int ip64 = 0, ipRL = 0;
+    if (nclu == 1) {
+        c0 = r_clu_rfields(a.clu[0]);
+        ip64 = a.ilist[min((int)(blockIdx.x * (NT / 64) + threadIdx.x / 64), a.npad - 1)];
+        ipRL = a.ilist[min((int)(blockIdx.x * (NT / RL) + threadIdx.x / RL), a.npad - 1)];
+    }
     const int total = a.glob->total_active;
     const bool wide = total <= R_WIDE_MAX;
     if (blockIdx.x * (wide ? NT / 64 : NT / RL) >= total) return;  // whole workgroup idle this step
@@ -957,8 +1109,8 @@ __global__ __launch_bounds__(NT) void r_reduce_correct_kernel(RArgs a)
     // one cluster and flushes it when the cluster changes
     double tacc = r_inf();
     int kacc = -1;
-    if (wide) r_reduce_body<ORDER, 64>(a, s_key, keys_lds, nclu, total, nb, c0, tacc, kacc);
-    else r_reduce_body<ORDER, RL>(a, s_key, keys_lds, nclu, total, nb, c0, tacc, kacc);
+    if (wide) r_reduce_body<ORDER, 64>(a, s_key, keys_lds, nclu, total, nb, c0, ip64, tacc, kacc);
+    else r_reduce_body<ORDER, RL>(a, s_key, keys_lds, nclu, total, nb, c0, ipRL, tacc, kacc);
     // one atomic per workgroup when all its lanes worked for one cluster
     __shared__ int s_k[NT / 64];
     __shared__ double s_t[NT / 64];
@@ -1260,7 +1412,7 @@ __global__ __launch_bounds__(NT) void r_fused_correct_kernel(RArgs a, int n, int
     const double told = in ? a.tn[off + i] : r_inf();
     RRec<RW> rr;
 #pragma unroll
-    for (int k = 0; k < RW; k++) rr.v[k] = in ? a.rp[(size_t)k * a.npad + off + i] : 0.0;
+    for (int k = 0; k < RW; k++) rr.v[k] = in && r_used(ORDER, k) ? a.rp[(size_t)k * a.npad + off + i] : 0.0;
     double f[NC];
 #pragma unroll
     for (int q = 0; q < NC; q++) f[q] = 0.0;
@@ -1284,7 +1436,7 @@ __global__ __launch_bounds__(NT) void r_fused_correct_kernel(RArgs a, int n, int
         a.tn[off + i] = tnew;
 #pragma unroll
         for (int k = 0; k < RW; k++)
-            if (k != 3 && k != R_ID) a.rp[(size_t)k * a.npad + off + i] = rr.v[k];
+            if (r_used(ORDER, k)) a.rp[(size_t)k * a.npad + off + i] = rr.v[k];
         if (!isfinite(rr.v[0] + rr.v[1] + rr.v[2] + rr.v[4] + rr.v[5] + rr.v[6])) atomicExch(&a.glob->error, 2);
     }
     // this workgroup's active count, and the minimum of its new block times
