@@ -758,9 +758,9 @@ struct hhr_ctx {
     // Optional per-step timing: four events per step of a batch (step start,
     // before and after the force launch, step end).  Timed runs launch their
     // steps directly on the stream (no graph), so the events are ordinary
-    // stream events.  The plan kernel also writes each step's active count
-    // to steplog, so every timed step is logged as (active count, force ms,
-    // ms of the other three launches).
+    // stream events.  The plan also writes each step's active count to
+    // steplog, so every timed step is logged as (active count, force ms,
+    // ms of the step's other launches).
     bool timing = false;
     std::vector<hipEvent_t> kev;
     int *steplog = nullptr;
@@ -813,7 +813,8 @@ static hipError_t r_mark(hhr_ctx *c, int k)
     return c->timing ? hipEventRecord(c->kev[k], c->stream) : hipSuccess;
 }
 
-// One block step: four launches; s is the step's index within its batch.
+// One block step: three launches (four for an ensemble); s is the step's
+// index within its batch.
 template <int ORDER, int MODE>
 static hipError_t r_launch_step(hhr_ctx *c, int s)
 {

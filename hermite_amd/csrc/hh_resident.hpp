@@ -135,14 +135,6 @@ __device__ __forceinline__ double r_pow_nth_quant(double x)
     return __longlong_as_double((long long)(e << 52));
 }
 
-struct RPart {  // one particle's words, loaded from / stored to the SoA record
-    double *rp;
-    int npad, i;
-    __device__ double &w(int k) const { return rp[(size_t)k * npad + i]; }
-    __device__ D3 v3(int k) const { return D3(w(k), w(k + 1), w(k + 2)); }
-    __device__ void set3(int k, const D3 &v) const { w(k) = v.x; w(k + 1) = v.y; w(k + 2) = v.z; }
-};
-
 // Record words the corrector of an order reads or writes (r_correct): 4th
 // order pos vel tlast acc jrk snp crk and dt; 6th order also d4a d5a; 8th
 // order every word but the mass and the id, which it never touches.
@@ -151,8 +143,8 @@ __host__ __device__ constexpr bool r_used(int o, int k)
     return k != 3 && k != R_ID && (o == 8 || k == R_DT || k < (o == 4 ? 20 : 26));
 }
 
-// One particle's words held in registers (loaded and stored by the caller),
-// with the accessors of RPart.
+// One particle's SoA record words held in registers (loaded and stored by
+// the caller).
 template <int RW>
 struct RRec {
     mutable double v[RW];
@@ -161,8 +153,8 @@ struct RRec {
     __device__ void set3(int k, const D3 &x) const { v[k] = x.x; v[k + 1] = x.y; v[k + 2] = x.z; }
 };
 
-// f: acc jrk [snp [crk]] of this particle.  Sets the new dt.  P: RPart
-// (the SoA record in memory) or RRec (the words in registers).
+// f: acc jrk [snp [crk]] of this particle.  Sets the new dt.  P: RRec
+// (the record's words in registers).
 template <int ORDER, class P>
 __device__ void r_correct(const P &p, const double *f, double etapow, double dtlim)
 {
@@ -310,17 +302,8 @@ __device__ __forceinline__ double r_wave_min(double x)
 
 __device__ __forceinline__ unsigned long long r_bits(double t) { return (unsigned long long)__double_as_longlong(t); }
 
-// Current block time of a cluster: the minimum over its slots, in every
-// lane of the calling wave.
-__device__ __forceinline__ double r_tmin_wave(const RClu &c, int p)
-{
-    const int lane = threadIdx.x & 63;
-    const double t = lane < R_SLOTS ? __longlong_as_double((long long)c.tmin_slot[p][lane]) : r_inf();
-    return r_wave_min(t);
-}
-
 template <int ORDER>
-__device__ void r_plan_body(const RArgs &a, bool atomic_counts);
+__device__ void r_plan_body(const RArgs &a);
 
 // Fields of a single cluster that its plan reads, loaded by every workgroup
 // of the compaction in its first round trip, so that the last workgroup
@@ -358,8 +341,10 @@ __device__ void r_plan_one(const RArgs &a, RClu &c, const RPlanIn &in, int step,
 // reserves the tile's list positions and counts the workgroup done; the
 // last workgroup to finish plans the step from registers (r_plan_one).
 // That saves the plan kernel's launch and every dependent read but the
-// atomic.  Ensembles append with one atomic per tile and plan in
-// r_plan_kernel.
+// atomic.  Ensembles append with one atomic per tile to their cluster's
+// counter and plan in r_plan_kernel (planning in the last workgroup, with
+// the workgroups counting themselves done in two levels of 64 counters,
+// made the 256-cluster ensemble 2.7 % slower on MI355X).
 template <int ORDER, bool EX, bool ONE>
 __global__ __launch_bounds__(NT) void r_compact_kernel(RArgs a)
 {
@@ -536,7 +521,7 @@ __device__ __forceinline__ RGeom r_geometry(const RPlan &pl, bool exact, int nac
 // proportion to each cluster's nact * n work; task, active-list and partial
 // offsets are block scans in cluster order.
 template <int ORDER>
-__device__ void r_plan_body(const RArgs &a, bool atomic_counts)
+__device__ void r_plan_body(const RArgs &a)
 {
     constexpr int NC = f_words(ORDER);
     constexpr int NW = NT / 64;
@@ -555,10 +540,7 @@ __device__ void r_plan_body(const RArgs &a, bool atomic_counts)
     double w = 0.0;
     for (int k = k0; k < k1; k++) {
         RClu &c = a.clu[k];
-        // counts appended by workgroups of the running launch are read as
-        // atomics (from the point where those atomics were performed)
-        c.nact = atomic_counts ? __hip_atomic_load(&c.acnt[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                               : c.acnt[p];
+        c.nact = c.acnt[p];
         c.acnt[p] = 0;
         c.mode = 0;
         c.ntask = 0;
@@ -639,7 +621,7 @@ __device__ void r_plan_body(const RArgs &a, bool atomic_counts)
 template <int ORDER>
 __global__ __launch_bounds__(NT) void r_plan_kernel(RArgs a)
 {
-    r_plan_body<ORDER>(a, false);
+    r_plan_body<ORDER>(a);
 }
 
 // The plan of one cluster's block step (r_plan_body for nclu = 1, with the
