@@ -22,7 +22,8 @@ starts from t = 0.
 
 value = interactions/s of the plain FP64 kernel over the timed run
 (sum of nact * N over block steps / wall), with |dE/E| at its end.  A second,
-event-timed pass of the same integration gives the force kernel's mean
+timed pass of the same integration (the device clock written at every
+kernel's start, in the same hipGraph replay) gives the force kernel's mean
 launch time for the roofline and the time split by active count.  Beside
 it: the split-precision kernel's integration outcome, both kernels'
 all-pairs force rate (the reference's "Init N square" evaluation), the split
@@ -109,7 +110,9 @@ def integrate(w, variant, states, tend, timing=False, graph=False, chunks=1, wal
 
     try:
         if timing:
-            r.set_timing(True)
+            # device clock stamps at kernel starts: the steps still replay
+            # from the graph (direct launches with --direct)
+            r.set_timing(True, stamps=True)
         if max_bsteps:
             r.set_step_limit(max_bsteps)
         ends = [tend * k / chunks for k in range(1, chunks + 1)]
@@ -119,7 +122,7 @@ def integrate(w, variant, states, tend, timing=False, graph=False, chunks=1, wal
         try:
             for t in ends:
                 t0 = time.perf_counter()
-                r.run(t, use_graph=graph and not timing, batch=BATCH)
+                r.run(t, use_graph=graph, batch=BATCH)
                 wall += time.perf_counter() - t0
                 if len(ends) > 1:
                     log(f"  {variant} t={t:g}: {wall:.1f} s")
@@ -367,7 +370,9 @@ def cpu_baseline(w, states, cpu_tend, budget_s=20.0):
 
 
 def by_active_count(log_, n_j):
-    """Force-kernel time split by the step's active count (events)."""
+    """Step time split by the step's active count: force (force kernel start
+    to corrector start) and the step's other launches, from device clock
+    stamps."""
     na, fm, om = log_
     out = []
     for lo, hi in ((1, 64), (65, 1024), (1025, 16384), (16385, 1 << 40)):
@@ -380,8 +385,9 @@ def by_active_count(log_, n_j):
 
 
 def kernel_rate(ft, order):
-    """Mean event-timed force launch (ms) and the conventional TF/s of a
-    launch (interactions x 60/97/144 flops per launch / its mean time)."""
+    """Mean stamp-timed force launch (ms: force kernel start to corrector
+    start) and the conventional TF/s of a launch (interactions x 60/97/144
+    flops per launch / its mean time)."""
     launches = max(ft["force_launches"], 1)
     avg_ms = ft["force_ms"] / launches
     flops_per_launch = ft["interactions"] * H.FLOPS[order] / launches
@@ -467,8 +473,8 @@ def main():
     ap.add_argument("--direct", action="store_true",
                     help="launch the timed steps directly on the stream instead of replaying a hipGraph")
     ap.add_argument("--single-pass", action="store_true",
-                    help="time the event-timed pass itself instead of a separate untimed run (default for the "
-                         "long workloads c3 and c5; the events add about 10 us per block step)")
+                    help="time the stamp-timed pass itself instead of a separate untimed run (default for the "
+                         "long workloads c3 and c5; the stamps add under 1 %% to a run)")
     ap.add_argument("--no-split", action="store_true", help="skip the split-precision legs")
     ap.add_argument("--no-micro", action="store_true", help="skip the single-force-call microbenchmark")
     ap.add_argument("--micro", action="store_true",
@@ -517,7 +523,7 @@ def main():
     single = args.single_pass or w.name in ("c3", "c5")
     chunks = 16 if w.name in ("c3", "c5") else 1
     t_cfg = w.tend  # the workload's own end time: |dE/E| is reported there too
-    log(f"timed: FP64 resident integration t=0->{tend:g}" + (" (event-timed)" if single else ""))
+    log(f"timed: FP64 resident integration t=0->{tend:g}" + (" (stamp-timed)" if single else ""))
     f = integrate(w, "fp64", states, tend, timing=single, graph=graph, chunks=chunks, t_energy=t_cfg)
     if f["error"]:
         raise SystemExit(f"FP64 integration failed: {f['error']}")
@@ -526,15 +532,15 @@ def main():
     if single:
         ft = f
     else:
-        log("event-timed pass of the same integration (roofline)")
-        ft = integrate(w, "fp64", states, tend, timing=True)
+        log("stamp-timed pass of the same integration (roofline)")
+        ft = integrate(w, "fp64", states, tend, timing=True, graph=graph)
     avg_ms, achieved = kernel_rate(ft, w.order)
     split, ap2, split_roof = None, None, None
     if not args.no_split:
         # same span, same checks; a run that collapses or crawls (steps
         # pinned at dt_min) is stopped at 3x the FP64 wall or 4x its block
         # steps and reported as such
-        log(f"split-precision integration t=0->{tend:g}" + (" (event-timed)" if single else ""))
+        log(f"split-precision integration t=0->{tend:g}" + (" (stamp-timed)" if single else ""))
         s = integrate(w, "split", states, tend, timing=single, graph=graph, chunks=chunks, t_energy=t_cfg,
                       wall_limit=max(60.0, 3 * f["wall_s"]), max_bsteps=4 * f["num_bstep"] + 1024)
         log(f"split: t={s['tsys']:.3e} after {s['num_bstep']} block steps, {s['wall_s']:.3f} s: "
@@ -547,7 +553,7 @@ def main():
                  f"dE_E_at_t{t_cfg:g}_over_fp64": (None if s.get("dE_E_at_t_energy") is None
                                                   else s["dE_E_at_t_energy"] / f["dE_E_at_t_energy"])}
         if s["error"] is None:
-            st_ = s if single else integrate(w, "split", states, tend, timing=True)
+            st_ = s if single else integrate(w, "split", states, tend, timing=True, graph=graph)
             s_ms, s_ach = kernel_rate(st_, w.order)
             split_roof = split_roofline(w, s_ms, s_ach, st_["force_launches"])
         if len(states) == 1:
@@ -599,7 +605,9 @@ def main():
         "micro": mic,
         "cpu_baseline": cpu,
     }
-    line["timing"] = "one event-timed pass" if single else "untimed run, then an event-timed pass for the roofline"
+    line["timing"] = ("one stamp-timed pass (device clock at every kernel start, graph replay)" if single else
+                      "untimed run, then a stamp-timed pass (device clock at every kernel start, graph replay) "
+                      "for the roofline")
     if w.name == "c4":
         line["dE_E_median"] = f.get("dE_E_median")
     print(json.dumps(line), flush=True)

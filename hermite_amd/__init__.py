@@ -521,10 +521,13 @@ class Resident:
         self._check(hip_lib().hhr_set_fused(self.h, int(max_n), int(grid), int(jpl), int(islice_max)),
                     "hhr_set_fused")
 
-    def set_timing(self, on=True):
-        """Bracket every force launch with HIP events (resets the totals);
-        timed runs launch their steps without a graph."""
-        self._check(hip_lib().hhr_set_timing(self.h, int(on)), "hhr_set_timing")
+    def set_timing(self, on=True, stamps=False):
+        """Time every block step (resets the totals).  stamps=False: HIP
+        events around direct launches (run() then launches without a
+        graph); stamps=True: the device clock written at every kernel's
+        start, inside replayed graphs as well."""
+        mode = (2 if stamps else 1) if on else 0
+        self._check(hip_lib().hhr_set_timing(self.h, mode), "hhr_set_timing")
 
     def force_timing(self):
         """(summed force-kernel device ms, launches) since set_timing."""

@@ -761,10 +761,16 @@ struct hhr_ctx {
     // stream events.  The plan also writes each step's active count to
     // steplog, so every timed step is logged as (active count, force ms,
     // ms of the step's other launches).
-    bool timing = false;
+    // Mode 2 times the same steps inside replayed graphs instead: the first
+    // workgroup of every kernel writes the device clock at kernel start
+    // (stamp, 4 words per step + the end of the batch), converted with the
+    // device's wall-clock rate.
+    int timing = 0;  // 0 off, 1 events around direct launches, 2 device clock stamps
     std::vector<hipEvent_t> kev;
     int *steplog = nullptr;
     int steplog_cap = 0;
+    long long *stamp = nullptr;
+    double wall_khz = 0.0;
     double force_ms = 0.0;
     long long force_launches = 0;
     std::vector<int> log_nact;
@@ -810,29 +816,35 @@ static RArgs r_args(hhr_ctx *c)
 
 static hipError_t r_mark(hhr_ctx *c, int k)
 {
-    return c->timing ? hipEventRecord(c->kev[k], c->stream) : hipSuccess;
+    return c->timing == 1 ? hipEventRecord(c->kev[k], c->stream) : hipSuccess;
 }
 
 // One block step: three launches (four for an ensemble); s is the step's
-// index within its batch.
-template <int ORDER, int MODE>
-static hipError_t r_launch_step(hhr_ctx *c, int s)
+// index within its batch.  ST: the kernels' timed instances (timing mode 2).
+template <int ORDER, int MODE, bool ST>
+static hipError_t r_launch_step_t(hhr_ctx *c, int s)
 {
     const RArgs a = r_args(c);
     hipError_t e = r_mark(c, 4 * s);
     if (e != hipSuccess) return e;
     if (c->nclu == 1) {  // one cluster: planned by the compaction's last workgroup
-        hipLaunchKernelGGL((r_compact_kernel<ORDER, MODE == EXACT, true>), dim3(c->ntiles), dim3(NT), 0, c->stream, a);
+        hipLaunchKernelGGL((r_compact_kernel<ORDER, MODE == EXACT, true, ST>), dim3(c->ntiles), dim3(NT), 0, c->stream, a);
     } else {
-        hipLaunchKernelGGL((r_compact_kernel<ORDER, MODE == EXACT, false>), dim3(c->ntiles), dim3(NT), 0, c->stream, a);
+        hipLaunchKernelGGL((r_compact_kernel<ORDER, MODE == EXACT, false, ST>), dim3(c->ntiles), dim3(NT), 0, c->stream, a);
         hipLaunchKernelGGL((r_plan_kernel<ORDER>), dim3(1), dim3(NT), 0, c->stream, a);
     }
     if ((e = r_mark(c, 4 * s + 1)) != hipSuccess) return e;
-    hipLaunchKernelGGL((r_force_kernel<ORDER, MODE>), dim3(c->force_grid), dim3(NT), 0, c->stream, a);
+    hipLaunchKernelGGL((r_force_kernel<ORDER, MODE, ST>), dim3(c->force_grid), dim3(NT), 0, c->stream, a);
     if ((e = r_mark(c, 4 * s + 2)) != hipSuccess) return e;
-    hipLaunchKernelGGL((r_reduce_correct_kernel<ORDER>), dim3(c->reduce_grid), dim3(NT), 0, c->stream, a);
+    hipLaunchKernelGGL((r_reduce_correct_kernel<ORDER, ST>), dim3(c->reduce_grid), dim3(NT), 0, c->stream, a);
     if ((e = r_mark(c, 4 * s + 3)) != hipSuccess) return e;
     return hipGetLastError();
+}
+
+template <int ORDER, int MODE>
+static hipError_t r_launch_step(hhr_ctx *c, int s)
+{
+    return c->timing == 2 ? r_launch_step_t<ORDER, MODE, true>(c, s) : r_launch_step_t<ORDER, MODE, false>(c, s);
 }
 
 // One block step of a small single cluster in two launches (list + force,
@@ -840,7 +852,7 @@ static hipError_t r_launch_step(hhr_ctx *c, int s)
 // Its partials (one per j-slice and slot) fit in the partial buffer, which
 // holds at least NT * R_TASK_MAX force records.
 static_assert((size_t)R_FUSED_SLICES * R_FUSED_MAX <= (size_t)NT * R_TASK_MAX, "fused partials");
-template <int ORDER, int MODE, int JPL>
+template <int ORDER, int MODE, int JPL, bool ST>
 static hipError_t r_launch_fused_jpl(hhr_ctx *c, int s)
 {
     const RArgs a = r_args(c);
@@ -856,13 +868,13 @@ static hipError_t r_launch_fused_jpl(hhr_ctx *c, int s)
     if (islice) {
         const int Si = (n + NT - 1) / NT;
         const int wpc = std::max(1, c->fused_grid / (Si * S));  // workgroups per (i-slice, j-slice)
-        hipLaunchKernelGGL((r_fused_islice_kernel<ORDER, MODE, JPL>), dim3(wpc * Si * S), dim3(NT), 0, c->stream, a, n, off, Si, S,
+        hipLaunchKernelGGL((r_fused_islice_kernel<ORDER, MODE, JPL, ST>), dim3(wpc * Si * S), dim3(NT), 0, c->stream, a, n, off, Si, S,
                            wpc);
     } else {
-        hipLaunchKernelGGL((r_fused_force_kernel<ORDER, MODE, JPL>), dim3(wps * S), dim3(NT), 0, c->stream, a, n, off, S, wps);
+        hipLaunchKernelGGL((r_fused_force_kernel<ORDER, MODE, JPL, ST>), dim3(wps * S), dim3(NT), 0, c->stream, a, n, off, S, wps);
     }
     if ((e = r_mark(c, 4 * s + 2)) != hipSuccess) return e;
-    hipLaunchKernelGGL((r_fused_correct_kernel<ORDER>), dim3((n + NT - 1) / NT), dim3(NT), 0, c->stream, a, n, off, S);
+    hipLaunchKernelGGL((r_fused_correct_kernel<ORDER, ST>), dim3((n + NT - 1) / NT), dim3(NT), 0, c->stream, a, n, off, S);
     if ((e = r_mark(c, 4 * s + 3)) != hipSuccess) return e;
     return hipGetLastError();
 }
@@ -875,9 +887,14 @@ static hipError_t r_launch_fused_step(hhr_ctx *c, int s)
 {
     const int n = c->n[0];
     const int jpl = c->fused_jpl ? c->fused_jpl : (n <= 1024 || ORDER == 8) ? 2 : 4;
-    if (jpl == 1) return r_launch_fused_jpl<ORDER, MODE, 1>(c, s);
-    if (jpl == 2) return r_launch_fused_jpl<ORDER, MODE, 2>(c, s);
-    return r_launch_fused_jpl<ORDER, MODE, 4>(c, s);
+    if (c->timing == 2) {
+        if (jpl == 1) return r_launch_fused_jpl<ORDER, MODE, 1, true>(c, s);
+        if (jpl == 2) return r_launch_fused_jpl<ORDER, MODE, 2, true>(c, s);
+        return r_launch_fused_jpl<ORDER, MODE, 4, true>(c, s);
+    }
+    if (jpl == 1) return r_launch_fused_jpl<ORDER, MODE, 1, false>(c, s);
+    if (jpl == 2) return r_launch_fused_jpl<ORDER, MODE, 2, false>(c, s);
+    return r_launch_fused_jpl<ORDER, MODE, 4, false>(c, s);
 }
 
 static bool r_fused(const hhr_ctx *c)
@@ -917,6 +934,26 @@ static void r_free_events(hhr_ctx *c)
     (void)hipFree(c->steplog);
     c->steplog = nullptr;
     c->steplog_cap = 0;
+    (void)hipFree(c->stamp);
+    c->stamp = nullptr;
+}
+
+// A graph captured with other kernel arguments (timing buffers) is stale.
+static void r_drop_graph(hhr_ctx *c)
+{
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->gexec = nullptr;
+    c->graph = nullptr;
+    c->graph_steps = 0;
+}
+
+// The clock at the end of a batch (timing mode 2): its last step's end.
+static hipError_t r_stamp_end(hhr_ctx *c)
+{
+    if (c->timing != 2) return hipSuccess;
+    hipLaunchKernelGGL(r_stamp_end_kernel, dim3(1), dim3(1), 0, c->stream, r_args(c));
+    return hipGetLastError();
 }
 
 static void r_free(hhr_ctx *c)
@@ -1115,6 +1152,12 @@ int hhr_create(int order, int variant, int device, int nclu, const int *ns, cons
     // force tasks (up to the 8192 of the plan's target) mostly get a
     // workgroup each.  Measured faster than 4 per CU (C2 1-2 %, C4 4 %) and
     // no slower than 32
+    {
+        int khz = 0;
+        if ((e = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device)) != hipSuccess)
+            return bail("wall clock rate", e);
+        c->wall_khz = khz;
+    }
     c->force_grid = prop.multiProcessorCount * 16;
     c->reduce_grid = prop.multiProcessorCount * 16;
     *out = c;
@@ -1134,21 +1177,30 @@ int hhr_destroy(hhr_ctx *c)
 // launched in batches of batch_steps (captured once into a hipGraph when
 // use_graph != 0); after each batch the number of clusters still active is
 // read back.  Extra steps after the last cluster finishes are no-ops.
-// With force timing on, steps are launched directly (use_graph ignored).
+// With force timing in mode 1, steps are launched directly (use_graph
+// ignored); mode 2 times graph replays (and direct launches) from device
+// clock stamps.
 int hhr_run(hhr_ctx *c, double tend, int use_graph, int batch_steps)
 {
     if (!c || batch_steps <= 0) return HH_E_ARG;
-    if (c->timing) use_graph = 0;
+    if (c->timing == 1) use_graph = 0;
     RCHK(c, hipSetDevice(c->device));
     // set tend for every cluster
     std::vector<RClu> hc(c->nclu);
     RCHK(c, hipMemcpy(hc.data(), c->clu, sizeof(RClu) * c->nclu, hipMemcpyDeviceToHost));
     for (auto &h : hc) h.tend = tend;
     RCHK(c, hipMemcpy(c->clu, hc.data(), sizeof(RClu) * c->nclu, hipMemcpyHostToDevice));
-    if (c->timing && (int)c->kev.size() != 4 * batch_steps) {
+    if (c->timing && c->steplog_cap != batch_steps) {
         r_free_events(c);
-        c->kev.assign(4 * batch_steps, nullptr);
-        for (auto &e : c->kev) RCHK(c, hipEventCreate(&e));
+        r_drop_graph(c);
+        if (c->timing == 1) {
+            c->kev.assign(4 * batch_steps, nullptr);
+            for (auto &e : c->kev) RCHK(c, hipEventCreate(&e));
+        } else {
+            RCHK(c, hipMalloc(&c->stamp, sizeof(long long) * (4 * (size_t)batch_steps + 1)));
+            RCHK(c, hipMemcpy((char *)c->glob + offsetof(RGlob, stamp), &c->stamp, sizeof(c->stamp),
+                              hipMemcpyHostToDevice));
+        }
         RCHK(c, hipMalloc(&c->steplog, sizeof(int) * batch_steps));
         c->steplog_cap = batch_steps;
     }
@@ -1160,6 +1212,7 @@ int hhr_run(hhr_ctx *c, double tend, int use_graph, int batch_steps)
         RCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         hipError_t e = hipSuccess;
         for (int s = 0; s < batch_steps && e == hipSuccess; s++) e = r_step(c, s);
+        if (e == hipSuccess) e = r_stamp_end(c);
         hipGraph_t gph = nullptr;
         const hipError_t e2 = hipStreamEndCapture(c->stream, &gph);
         if (e != hipSuccess) return rfail(c, HH_E_HIP, std::string("step capture: ") + hipGetErrorString(e));
@@ -1179,6 +1232,8 @@ int hhr_run(hhr_ctx *c, double tend, int use_graph, int batch_steps)
                 const hipError_t e = r_step(c, s);
                 if (e != hipSuccess) return rfail(c, HH_E_HIP, std::string("step launch: ") + hipGetErrorString(e));
             }
+            const hipError_t e = r_stamp_end(c);
+            if (e != hipSuccess) return rfail(c, HH_E_HIP, std::string("stamp launch: ") + hipGetErrorString(e));
         }
         c->steps_launched += batch_steps;
         launched += batch_steps;
@@ -1187,11 +1242,30 @@ int hhr_run(hhr_ctx *c, double tend, int use_graph, int batch_steps)
         if (c->timing) {
             std::vector<int> na(batch_steps);
             RCHK(c, hipMemcpy(na.data(), c->steplog, sizeof(int) * batch_steps, hipMemcpyDeviceToHost));
+            std::vector<long long> ts;
+            if (c->timing == 2) {
+                ts.resize(4 * (size_t)batch_steps + 1);
+                RCHK(c, hipMemcpy(ts.data(), c->stamp, sizeof(long long) * ts.size(), hipMemcpyDeviceToHost));
+            }
             for (int s = 0; s < batch_steps; s++) {
                 float t01 = 0.f, t12 = 0.f, t23 = 0.f;
-                RCHK(c, hipEventElapsedTime(&t01, c->kev[4 * s], c->kev[4 * s + 1]));
-                RCHK(c, hipEventElapsedTime(&t12, c->kev[4 * s + 1], c->kev[4 * s + 2]));
-                RCHK(c, hipEventElapsedTime(&t23, c->kev[4 * s + 2], c->kev[4 * s + 3]));
+                if (c->timing == 1) {
+                    RCHK(c, hipEventElapsedTime(&t01, c->kev[4 * s], c->kev[4 * s + 1]));
+                    RCHK(c, hipEventElapsedTime(&t12, c->kev[4 * s + 1], c->kev[4 * s + 2]));
+                    RCHK(c, hipEventElapsedTime(&t23, c->kev[4 * s + 2], c->kev[4 * s + 3]));
+                } else {
+                    // step s of the batch is step g.step - batch_steps + s, whose
+                    // stamps sit at that step modulo the batch size; the next
+                    // step's first stamp (or the batch's end) closes it
+                    const long long st = (long long)g.step - batch_steps + s;
+                    const size_t k = (size_t)(st % batch_steps);
+                    const long long t_end = s + 1 < batch_steps ? ts[(size_t)((st + 1) % batch_steps) * 4]
+                                                                : ts[4 * (size_t)batch_steps];
+                    const double ms_per_tick = 1.0 / c->wall_khz;
+                    t01 = (float)((ts[k * 4 + 2] - ts[k * 4]) * ms_per_tick);
+                    t12 = (float)((ts[k * 4 + 3] - ts[k * 4 + 2]) * ms_per_tick);
+                    t23 = (float)((t_end - ts[k * 4 + 3]) * ms_per_tick);
+                }
                 c->force_ms += t12;
                 // steps batch_steps before g.step wrote their counts at (step % cap)
                 const long long st = (long long)g.step - batch_steps + s;
@@ -1292,13 +1366,14 @@ int hhr_set_step_limit(hhr_ctx *c, long long max_block_steps)
 
 int hhr_set_timing(hhr_ctx *c, int on)
 {
-    if (!c) return HH_E_ARG;
-    if (c->timing != (on != 0)) {
+    if (!c || on < 0 || on > 2) return HH_E_ARG;
+    if (c->timing != on) {
         RCHK(c, hipSetDevice(c->device));
         RCHK(c, hipStreamSynchronize(c->stream));
         r_free_events(c);
+        r_drop_graph(c);
     }
-    c->timing = on != 0;
+    c->timing = on;
     c->force_ms = 0.0;
     c->force_launches = 0;
     c->log_nact.clear();

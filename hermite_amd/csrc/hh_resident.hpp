@@ -88,6 +88,10 @@ struct RGlob {
     // one cluster's compaction: workgroups done (high 32 bits) and active
     // particles appended (low 32 bits) in one word, parity buffered
     unsigned long long cpack[2];
+    // timing mode 2: kernel start clocks of step s at stamp[(s % cap) * 4 + k]
+    // (read by the timed kernel instances only, so the untimed ones keep
+    // their kernel arguments)
+    long long *stamp;
 };
 
 struct RArgs {
@@ -106,6 +110,36 @@ struct RArgs {
     int *__restrict__ steplog;   // optional: active count of step s at [s % steplog_cap]
     int steplog_cap;
 };
+
+// Step timing inside replayed graphs (hhr_set_timing mode 2): the first
+// thread of the first workgroup of each kernel of block step s writes the
+// device's constant-rate clock, read at kernel start, to
+// stamp[(s % cap) * 4 + k]: k = 0 compaction (the fused force kernel: the
+// step's first launch), 2 force, 3 corrector.  The launches of a graph
+// follow each other with no host in between, so the next step's first
+// stamp ends a step.  Kernels that may advance the step counter themselves
+// pass the value they read before their own atomics.  Each kernel has a
+// timed instance (template flag STAMP) and an untimed one without this
+// code, which would change the untimed kernels' register allocation and
+// cost them 1-3 % (measured).
+
+__device__ __forceinline__ void r_stamp(const RArgs &a, int k, int step, long long t)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.glob->stamp[(size_t)(step % a.steplog_cap) * 4 + k] = t;
+}
+
+// For kernels during which nothing changes the step counter (back = 1
+// after the plan has advanced it): the clock is read by the one thread that
+// stores it.
+__device__ __forceinline__ void r_stamp_glob(const RArgs &a, int k, int back)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const long long t = wall_clock64();
+        long long *st = a.glob->stamp + (size_t)((a.glob->step - back) % a.steplog_cap) * 4;
+        st[k] = t;
+        if (k == 0) st[2] = t;
+    }
+}
 
 __device__ __forceinline__ double r_inf() { return __longlong_as_double(0x7ff0000000000000LL); }
 
@@ -345,9 +379,10 @@ __device__ void r_plan_one(const RArgs &a, RClu &c, const RPlanIn &in, int step,
 // counter and plan in r_plan_kernel (planning in the last workgroup, with
 // the workgroups counting themselves done in two levels of 64 counters,
 // made the 256-cluster ensemble 2.7 % slower on MI355X).
-template <int ORDER, bool EX, bool ONE>
+template <int ORDER, bool EX, bool ONE, bool STAMP>
 __global__ __launch_bounds__(NT) void r_compact_kernel(RArgs a)
 {
+    const long long t_start = STAMP ? wall_clock64() : 0;
     constexpr int W = jp_words(ORDER), Q = pr_words(ORDER);
     constexpr int NW = NT / 64;
     __shared__ double wmin[NW];
@@ -359,6 +394,7 @@ __global__ __launch_bounds__(NT) void r_compact_kernel(RArgs a)
     RClu &c = a.clu[k];
     const int i = tile * TJ + tid;
     const int step = a.glob->step;
+    if constexpr (STAMP) r_stamp(a, 0, step, t_start);
     const unsigned long long none = 0x7ff0000000000000ULL;
     const unsigned long long b0 = lane < R_SLOTS ? c.tmin_slot[0][lane] : none;
     const unsigned long long b1 = lane < R_SLOTS ? c.tmin_slot[1][lane] : none;
@@ -616,6 +652,12 @@ __device__ void r_plan_body(const RArgs &a)
         if (a.steplog) a.steplog[a.glob->step % a.steplog_cap] = (int)tot_act;
         a.glob->step += 1;
     }
+}
+
+// End of a batch of timed steps: the clock after its last step.
+__global__ void r_stamp_end_kernel(RArgs a)
+{
+    a.glob->stamp[(size_t)a.steplog_cap * 4] = wall_clock64();
 }
 
 template <int ORDER>
@@ -984,9 +1026,13 @@ __device__ __forceinline__ void r_force_body(const RArgs &a)
 // Two waves per SIMD: the 8th-order body alone would take 288 VGPRs (one
 // wave); held to 256 with a few spills it runs 4 % faster.  Three waves
 // (168 VGPRs) made the long steps of the 6th-order integration 7 % slower.
-template <int ORDER, int MODE>
+//
+// STAMP: the timed instance writes the step's force stamp (hhr_set_timing
+// mode 2).
+template <int ORDER, int MODE, bool STAMP>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void r_force_kernel(RArgs a)
 {
+    if constexpr (STAMP) r_stamp_glob(a, 2, 1);
     r_force_body<ORDER, MODE>(a);
 }
 
@@ -1063,9 +1109,10 @@ __device__ __forceinline__ void r_reduce_body(const RArgs &a, const int *s_key, 
     }
 }
 
-template <int ORDER>
+template <int ORDER, bool STAMP>
 __global__ __launch_bounds__(NT) void r_reduce_correct_kernel(RArgs a)
 {
+    if constexpr (STAMP) r_stamp_glob(a, 3, 1);
     __shared__ int s_key[R_CLU_LDS];
     const int nclu = a.nclu;
     // a single cluster's fields, and this thread's first active-list entry
@@ -1174,7 +1221,7 @@ constexpr int R_FUSED_SLICES = R_FUSED_MAX / NT;  // at most, with one j per lan
 // minima of both parities loaded together with the step counter.
 struct RFusedStep {
     double tmin, dtlim;
-    int p;
+    int step, p;
     bool live, collapsed, stepping;
 };
 
@@ -1185,7 +1232,8 @@ __device__ __forceinline__ RFusedStep r_fused_step(const RArgs &a, const RClu &c
     const unsigned long long b0 = lane < R_SLOTS ? c.tmin_slot[0][lane] : none;
     const unsigned long long b1 = lane < R_SLOTS ? c.tmin_slot[1][lane] : none;
     RFusedStep f;
-    f.p = a.glob->step & 1;
+    f.step = a.glob->step;
+    f.p = f.step & 1;
     f.tmin = r_wave_min(__longlong_as_double((long long)(f.p ? b1 : b0)));
     f.live = f.tmin <= c.tend;
     f.dtlim = f.live ? r_dtlim(f.tmin, c.dtmax) : 0.0;
@@ -1236,9 +1284,10 @@ __device__ void r_fused_bookkeep(const RArgs &a, RClu &c, const RFusedStep &st, 
     a.glob->step += 1;
 }
 
-template <int ORDER, int MODE, int JPL>
+template <int ORDER, int MODE, int JPL, bool STAMP>
 __global__ __launch_bounds__(NT) void r_fused_force_kernel(RArgs a, int n, int off, int S, int wps)
 {
+    if constexpr (STAMP) r_stamp_glob(a, 0, 0);  // also the force stamp: the step's first launch
     static_assert(MODE != EXACT, "EXACT sums j in index order: three-launch step");
     constexpr int W = jp_words(ORDER), Q = pr_words(ORDER);
     constexpr int NC = f_words(ORDER);
@@ -1377,9 +1426,10 @@ __global__ __launch_bounds__(NT) void r_fused_force_kernel(RArgs a, int n, int o
     }
 }
 
-template <int ORDER>
+template <int ORDER, bool STAMP>
 __global__ __launch_bounds__(NT) void r_fused_correct_kernel(RArgs a, int n, int off, int S)
 {
+    const long long t_start = STAMP ? wall_clock64() : 0;
     constexpr int NC = f_words(ORDER), RW = r_words(ORDER);
     constexpr int NW = NT / 64;
     __shared__ double s_wmin[NW];
@@ -1404,6 +1454,7 @@ __global__ __launch_bounds__(NT) void r_fused_correct_kernel(RArgs a, int n, int
             for (int q = 0; q < NC; q++) f[q] += a.part[((size_t)sl * R_FUSED_MAX + i) * NC + q];
     const double etapow = c.etapow, dtmin = c.dtmin;
     const RFusedStep st = r_fused_step(a, c);
+    if constexpr (STAMP) r_stamp(a, 3, st.step, t_start);
     const bool act = st.stepping && told == st.tmin;
     double tnew = told;
     if (act) {  // r_finish on the words in registers
@@ -1465,9 +1516,10 @@ __global__ __launch_bounds__(NT) void r_fused_correct_kernel(RArgs a, int n, int
 // A group's i are taken from one i-slice, so active particles crowded
 // into few slots leave workgroups of the other i-slices idle; the kernel
 // of the global active list above balances those steps.
-template <int ORDER, int MODE, int JPL>
+template <int ORDER, int MODE, int JPL, bool STAMP>
 __global__ __launch_bounds__(NT) void r_fused_islice_kernel(RArgs a, int n, int off, int Si, int Sj, int wpc)
 {
+    if constexpr (STAMP) r_stamp_glob(a, 0, 0);  // also the force stamp: the step's first launch
     static_assert(MODE != EXACT, "EXACT sums j in index order: three-launch step");
     constexpr int W = jp_words(ORDER), Q = pr_words(ORDER);
     constexpr int NC = f_words(ORDER);

@@ -263,11 +263,14 @@ int hhr_set_split_waves(hhr_ctx *ctx, int wj4_max, int wj2_max);
  * in the j summation order only. */
 int hhr_set_fused(hhr_ctx *ctx, int max_n, int grid, int jpl, int islice_max);
 
-/* Force-kernel timing (no reference counterpart): when on, hhr_run launches
+/* Block-step timing (no reference counterpart).  on = 1: hhr_run launches
  * its steps directly on the context's stream (no graph; the kernels are the
- * same) with a HIP event pair around every force launch, and
- * hhr_force_timing reports the summed device milliseconds and the number of
- * launches since timing was switched on. */
+ * same) with a HIP event pair around every force launch.  on = 2: the first
+ * workgroup of each kernel of a step writes the device's constant-rate clock
+ * at kernel start, in replayed graphs too; a step's force time runs from the
+ * force kernel's start to the corrector's.  hhr_force_timing reports the
+ * summed force milliseconds and the number of launches since timing was
+ * switched on; on = 0 switches timing off. */
 int hhr_set_timing(hhr_ctx *ctx, int on);
 int hhr_force_timing(const hhr_ctx *ctx, double *ms_total, long long *launches);
 /* Per-step log of the timed steps since timing was switched on (no reference

@@ -152,25 +152,35 @@ def test_ensemble_fp64_matches_oracle_statistically():
     ens.close()
 
 
-@pytest.mark.parametrize("use_graph", [False, True])
-def test_resident_force_timing_changes_nothing(use_graph):
-    """Event timing around every force launch (timed runs launch without a
-    graph) leaves the integration bit-identical and counts every launch."""
+@pytest.mark.parametrize("use_graph,stamps,fused", [(False, False, True), (True, False, True), (False, True, True),
+                                                     (True, True, True), (True, True, False), (False, True, False)])
+def test_resident_force_timing_changes_nothing(use_graph, stamps, fused):
+    """Step timing (events around direct launches, or the device clock
+    written at every kernel's start, in replayed graphs too), on the
+    two-launch and the three-launch step, leaves the integration
+    bit-identical, counts every launch and logs every block step with a
+    positive force and other time."""
     order, n, eps2 = 6, 500, (1 / 32) ** 2
     o = initialised(order, n, 43, eps2, 0.4, 0.1, 1 / 16)
     P0 = o.ptcl()
-    print(f"resident timing graph={use_graph}", flush=True)
+    print(f"resident timing graph={use_graph} stamps={stamps} fused={fused}", flush=True)
     a = H.Resident(order, "fp64", [(P0, eps2, 0.4, 1 / 16, 0.0, 0.0)])
-    a.run(1 / 8, use_graph=use_graph, batch=8)
     b = H.Resident(order, "fp64", [(P0, eps2, 0.4, 1 / 16, 0.0, 0.0)])
-    b.set_timing(True)
+    if not fused:
+        a.set_fused(0)
+        b.set_fused(0)
+    a.run(1 / 8, use_graph=use_graph, batch=8)
+    b.set_timing(True, stamps=stamps)
     b.run(1 / 16, use_graph=use_graph, batch=8)
     b.run(1 / 8, use_graph=use_graph, batch=8)
     ms, launches = b.force_timing()
+    na, fm, om = b.step_log()
     assert a.stats() == b.stats()
     assert np.array_equal(a.particles(), b.particles())
     assert launches == b.steps_launched and launches >= b.stats()["num_bstep"]
     assert 0 < ms < 1e4
+    assert len(na) == b.stats()["num_bstep"] and (na > 0).all()
+    assert (fm > 0).all() and (om > 0).all() and fm.astype(np.float64).sum() <= ms * 1.0001 + 1e-6
     a.close()
     b.close()
 
