@@ -827,10 +827,12 @@ static hipError_t r_launch_step_t(hhr_ctx *c, int s)
     const RArgs a = r_args(c);
     hipError_t e = r_mark(c, 4 * s);
     if (e != hipSuccess) return e;
-    if (c->nclu == 1) {  // one cluster: planned by the compaction's last workgroup
-        hipLaunchKernelGGL((r_compact_kernel<ORDER, MODE == EXACT, true, ST>), dim3(c->ntiles), dim3(NT), 0, c->stream, a);
+    if (c->nclu == 1 && c->ntiles <= R_PACK_MAX) {  // one cluster: planned by the compaction's last workgroup
+        hipLaunchKernelGGL((r_compact_kernel<ORDER, MODE == EXACT, 1, ST>), dim3(c->ntiles), dim3(NT), 0, c->stream, a);
+    } else if (c->nclu == 1) {
+        hipLaunchKernelGGL((r_compact_kernel<ORDER, MODE == EXACT, 2, ST>), dim3(c->ntiles), dim3(NT), 0, c->stream, a);
     } else {
-        hipLaunchKernelGGL((r_compact_kernel<ORDER, MODE == EXACT, false, ST>), dim3(c->ntiles), dim3(NT), 0, c->stream, a);
+        hipLaunchKernelGGL((r_compact_kernel<ORDER, MODE == EXACT, 0, ST>), dim3(c->ntiles), dim3(NT), 0, c->stream, a);
         hipLaunchKernelGGL((r_plan_kernel<ORDER>), dim3(1), dim3(NT), 0, c->stream, a);
     }
     if ((e = r_mark(c, 4 * s + 1)) != hipSuccess) return e;

@@ -88,11 +88,21 @@ struct RGlob {
     // one cluster's compaction: workgroups done (high 32 bits) and active
     // particles appended (low 32 bits) in one word, parity buffered
     unsigned long long cpack[2];
+    // one cluster of more than R_PACK_MAX tiles: compaction workgroups done
+    // per group of tiles (tile mod R_CGRP) and groups done, parity buffered
+    unsigned int cgrp[2][64];
+    unsigned int ctop[2];
     // timing mode 2: kernel start clocks of step s at stamp[(s % cap) * 4 + k]
     // (read by the timed kernel instances only, so the untimed ones keep
     // their kernel arguments)
     long long *stamp;
 };
+constexpr int R_CGRP = 64;
+// Largest tile count whose compaction workgroups all count themselves on
+// one packed atomic: they reach it together, right after their first round
+// trip, and same-address atomics are serialised (4096 tiles, N = 2^20: the
+// compaction took 130 us instead of 85).
+constexpr int R_PACK_MAX = 1024;
 
 struct RArgs {
     double *__restrict__ rp;     // resident records, SoA
@@ -379,9 +389,12 @@ __device__ void r_plan_one(const RArgs &a, RClu &c, const RPlanIn &in, int step,
 // counter and plan in r_plan_kernel (planning in the last workgroup, with
 // the workgroups counting themselves done in two levels of 64 counters,
 // made the 256-cluster ensemble 2.7 % slower on MI355X).
-template <int ORDER, bool EX, bool ONE, bool STAMP>
+// CM: 0 ensemble, 1 one cluster of at most R_PACK_MAX tiles, 2 one larger
+// cluster.
+template <int ORDER, bool EX, int CM, bool STAMP>
 __global__ __launch_bounds__(NT) void r_compact_kernel(RArgs a)
 {
+    constexpr bool ONE = CM != 0, packed = CM == 1;
     const long long t_start = STAMP ? wall_clock64() : 0;
     constexpr int W = jp_words(ORDER), Q = pr_words(ORDER);
     constexpr int NW = NT / 64;
@@ -402,16 +415,19 @@ __global__ __launch_bounds__(NT) void r_compact_kernel(RArgs a)
     const double t = a.tn[i];
     RPlanIn pin{};
     if constexpr (ONE) pin = r_plan_in(a, c);
-    // One cluster: the record is read before the block time is known
-    // (slot i is inside the store, i < npad), but only after the loads
-    // above have been issued: loads complete in issue order for the wait
-    // counter, so the schedule's first step waits for those few words alone
-    // while the records stream in (at N = 2^18 they are 54 MB per step).
-    // Ensembles read only the records of clusters still live: most are
-    // done during the last block steps of a run.
+    // One cluster of at most R_PACK_MAX tiles: the record is read before
+    // the block time is known (slot i is inside the store, i < npad), but
+    // only after the loads above have been issued: loads complete in issue
+    // order for the wait counter, so the schedule's first step waits for
+    // those few words alone while the records stream in (at N = 2^18 they
+    // are 54 MB per step).  Larger clusters read them once the block time
+    // is known: issued all at once, the 168 MB of records of N = 2^20 made
+    // the compaction 139 us instead of 90.  Ensembles read only the records
+    // of clusters still live: most are done during the last block steps of
+    // a run.
     __builtin_amdgcn_sched_barrier(0);
     double r[W];
-    if constexpr (ONE) {
+    if constexpr (packed) {
 #pragma unroll
         for (int w = 0; w < W; w++) r[w] = a.rp[(size_t)w * a.npad + i];
     }
@@ -429,6 +445,9 @@ __global__ __launch_bounds__(NT) void r_compact_kernel(RArgs a)
     __syncthreads();
     // one append per tile (same-address atomics are serialised at the
     // memory side, so fewer of them is what matters)
+    // one cluster of at most R_PACK_MAX tiles: the packed atomic below;
+    // more tiles: the append reserves (tiles with active particles only)
+    // and the workgroups count themselves done in two levels at their end
     if (tid == 0) {
         int tot = 0;
         for (int w = 0; w < NW; w++) {
@@ -436,7 +455,7 @@ __global__ __launch_bounds__(NT) void r_compact_kernel(RArgs a)
             tot += wcnt[w];
         }
         if (m < r_inf()) atomicMin(&c.tmin_slot[p ^ 1][tile % R_SLOTS], r_bits(m));
-        if constexpr (ONE) {
+        if constexpr (packed) {
             const unsigned long long old = atomicAdd(&a.glob->cpack[p], (1ULL << 32) | (unsigned long long)tot);
             s_base = (int)(old & 0xffffffffULL);
             s_nact = s_base + tot;
@@ -446,7 +465,7 @@ __global__ __launch_bounds__(NT) void r_compact_kernel(RArgs a)
         }
     }
     double q[Q];
-    if constexpr (!ONE) {
+    if constexpr (!packed) {
         if (live) {
 #pragma unroll
             for (int w = 0; w < W; w++) r[w] = a.rp[(size_t)w * a.npad + i];
@@ -471,6 +490,28 @@ __global__ __launch_bounds__(NT) void r_compact_kernel(RArgs a)
     // the last workgroup's count returned the total: every other workgroup
     // has appended; the plan reads nothing else they wrote
     if constexpr (ONE) {
+        if constexpr (!packed) {
+            if (tid == 0) {
+                // this tile's append has been performed (its atomic
+                // returned) before the tile counts itself done
+                __builtin_amdgcn_s_waitcnt(0);
+                const int g = tile % R_CGRP;
+                const unsigned in_g = (gridDim.x - g + R_CGRP - 1) / R_CGRP;  // tiles of group g
+                bool last = false;
+                if (atomicAdd(&a.glob->cgrp[p][g], 1u) == in_g - 1) {
+                    a.glob->cgrp[p][g] = 0;  // next used two steps on
+                    last = atomicAdd(&a.glob->ctop[p], 1u) == (unsigned)R_CGRP - 1;
+                }
+                if (last) {
+                    // every append has been performed: read the count as an atomic
+                    s_nact = __hip_atomic_load(&c.acnt[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    c.acnt[p] = 0;
+                    a.glob->ctop[p] = 0;
+                }
+                s_last = last;
+            }
+            __syncthreads();
+        }
         if (s_last) r_plan_one<ORDER>(a, c, pin, step, s_nact, tmin);
     }
 }

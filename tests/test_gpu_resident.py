@@ -185,6 +185,33 @@ def test_resident_force_timing_changes_nothing(use_graph, stamps, fused):
     b.close()
 
 
+def test_resident_exact_large_cluster_vs_oracle_inplace():
+    """A single cluster of more than 1024 tiles (N = 262400) takes the
+    compaction path that reads the slot records once the block time is
+    known and counts its workgroups done in two levels (64 group counters,
+    then the groups): its first 24 block steps, EXACT, are bit-identical to
+    the oracle's in-place schedule from the same initial records."""
+    order, n = 4, 1025 * 256
+    eps2, eta, eta_s, dtmax = (4.0 / n) ** 2, 0.1, 0.01, 1 / 16
+    m, x, v = H.plummer(n, 71)
+    print(f"resident large cluster n={n}", flush=True)
+    s = H.System(order, H.make_particles(order, m, x, v), 0.0, eps2, eta, eta_s, dtmax, variant="exact")
+    s.init_step()
+    P0 = s.ptcl()
+    s.close()
+    r = H.Resident(order, "exact", [(P0, eps2, eta, dtmax, 0.0, 0.0)])
+    r.set_step_limit(24)
+    with pytest.raises(H.StepLimitReached):
+        r.run(dtmax, use_graph=True, batch=8)
+    st = r.stats()
+    o = O.System(order, P0, 0.0, eps2, eta, eta_s, dtmax)
+    o.integrate_until_inplace(st["tsys"])
+    print(f"24 block steps to t = {st['tsys']:.6e}, {st['num_step']} particle steps", flush=True)
+    assert st["num_bstep"] == 24 and o.num_bstep == 24 and o.num_step == st["num_step"]
+    assert np.array_equal(r.particles(), o.ptcl())
+    r.close()
+
+
 def test_resident_exact_salpeter_bitwise_vs_oracle_inplace():
     """Unequal (Salpeter) masses, 6th order: the resident EXACT schedule is
     bit-identical to the oracle's in-place schedule."""
