@@ -652,15 +652,49 @@ __global__ __launch_bounds__(NT) void tile_kernel(TileArgs a)
 // Sum of the j-slice partials part[(s*nc + c)*ipad + i] over s, for i in
 // [0, ni), written as fout[i*f + c] (c < f) or pout[i] (c == f).  Each sum
 // is split over RL = 8 lanes (lane l takes s = l, l + 8, ...) and the lanes
-// are combined by a fixed xor-shuffle tree, so the order of additions, and
-// the result, depend only on nslice.  The reference reduces its NJBLOCK
-// partials with a warp-shuffle tree too (SRC/hermite6-titan.cu:441-459).
+// are combined by a fixed DPP tree (lane_tree8), so the order of additions,
+// and the result, depend only on nslice.  (The reference reduces its NJBLOCK
+// partials with a warp-shuffle tree, SRC/hermite6-titan.cu:441-459.)
 constexpr int RL = 8;
+
+// Cross-lane moves through DPP: the data stay in the vector ALU (a shuffle,
+// ds_bpermute, goes through the LDS crossbar and back).  A double moves as
+// its two 32-bit halves; lanes of rows that ROW_MASK leaves out get old.
+// Controls (GFX9 DPP): quad permutations 1,0,3,2 and 2,3,0,1, the mirror of
+// each half row (lane i <-> 7 - i within 8) and of each row (i <-> 15 - i
+// within 16), and the broadcast of lane 15 (31) of a row into the rows that
+// follow.
+constexpr int DPP_QUAD_XOR1 = 0xB1, DPP_QUAD_XOR2 = 0x4E, DPP_ROW_HALF_MIRROR = 0x141, DPP_ROW_MIRROR = 0x140;
+constexpr int DPP_ROW_BCAST15 = 0x142, DPP_ROW_BCAST31 = 0x143;
+
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_f64(double old, double x)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    const unsigned long long o = (unsigned long long)__double_as_longlong(old);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp((int)(unsigned)o, (int)(unsigned)b, CTRL, ROW_MASK, 0xf, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp((int)(unsigned)(o >> 32), (int)(unsigned)(b >> 32), CTRL,
+                                                              ROW_MASK, 0xf, false);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// The value of lane 63, in every lane.
+__device__ __forceinline__ double lane63(double x)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, 63);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), 63);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Sum over each aligned group of 8 lanes, in every lane of the group: lane
+// pairs, then quads, then the two quads (every lane adds the same two
+// numbers at each stage, so all get the same sum).
 __device__ __forceinline__ double lane_tree8(double x)
 {
-    x += __shfl_xor(x, 4, 64);
-    x += __shfl_xor(x, 2, 64);
-    x += __shfl_xor(x, 1, 64);
+    x += dpp_f64<DPP_QUAD_XOR1, 0xf>(0.0, x);
+    x += dpp_f64<DPP_QUAD_XOR2, 0xf>(0.0, x);
+    x += dpp_f64<DPP_ROW_HALF_MIRROR, 0xf>(0.0, x);
     return x;
 }
 
@@ -688,7 +722,7 @@ __global__ __launch_bounds__(256) void slice_reduce_kernel(const double *__restr
 // (coalesced SoA loads straight from HBM, no LDS staging) and every lane
 // holds a group of GI i-particles (uniform across the workgroup, read with
 // scalar-friendly broadcast loads).  Per-lane sums are reduced across the
-// wave with xor-shuffles and across the 4 waves through LDS; the j-slices
+// wave with DPP moves (wave_sum) and across the 4 waves through LDS; the j-slices
 // are added by slice_reduce_kernel (deterministic).  FP64 and SPLIT bodies only: the EXACT variant must sum j
 // in index order and uses tile_kernel with one slice.
 struct JParArgs {
@@ -704,11 +738,18 @@ struct JParArgs {
     double *__restrict__ pout;
 };
 
+// Sum over the 64 lanes of a wave, in every lane: groups of 8 (lane_tree8),
+// rows of 16 (mirror), rows 0 + 1 and 2 + 3 (lane 15 of a row broadcast into
+// rows 1 and 3), then the two halves (lane 31 into rows 2 and 3): the total
+// (r3 + r2) + (r1 + r0) lands in lane 63 and is broadcast from there.  The
+// order of the additions is fixed.
 __device__ __forceinline__ double wave_sum(double x)
 {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, 64);
-    return x;
+    x = lane_tree8(x);
+    x += dpp_f64<DPP_ROW_MIRROR, 0xf>(0.0, x);
+    x += dpp_f64<DPP_ROW_BCAST15, 0xa>(0.0, x);
+    x += dpp_f64<DPP_ROW_BCAST31, 0xc>(0.0, x);
+    return lane63(x);
 }
 
 template <int ORDER, int MODE, int GI, bool POT>
@@ -784,7 +825,7 @@ __global__ __launch_bounds__(NT) void jpar_kernel(JParArgs a)
         for (int g = 0; g < GI; g++) split_fold<ORDER>(A[g], Af[g]);
     }
 
-    // workgroup sum in a fixed order: shuffle tree per wave, then waves 0..3
+    // workgroup sum in a fixed order: DPP tree per wave, then waves 0..3
 #pragma unroll
     for (int g = 0; g < GI; g++)
 #pragma unroll

@@ -337,11 +337,17 @@ __device__ double r_finish(const RArgs &a, const RCluR &c, int i, RRec<r_words(O
 }
 
 // ------------------------------------------------------------ schedule
+// Minimum over the 64 lanes of a wave, in every lane (the stages of
+// wave_sum; rows left out of a broadcast keep their own value).
 __device__ __forceinline__ double r_wave_min(double x)
 {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) x = fmin(x, __shfl_xor(x, off, 64));
-    return x;
+    x = fmin(x, dpp_f64<DPP_QUAD_XOR1, 0xf>(x, x));
+    x = fmin(x, dpp_f64<DPP_QUAD_XOR2, 0xf>(x, x));
+    x = fmin(x, dpp_f64<DPP_ROW_HALF_MIRROR, 0xf>(x, x));
+    x = fmin(x, dpp_f64<DPP_ROW_MIRROR, 0xf>(x, x));
+    x = fmin(x, dpp_f64<DPP_ROW_BCAST15, 0xa>(x, x));
+    x = fmin(x, dpp_f64<DPP_ROW_BCAST31, 0xc>(x, x));
+    return lane63(x);
 }
 
 __device__ __forceinline__ unsigned long long r_bits(double t) { return (unsigned long long)__double_as_longlong(t); }
@@ -629,7 +635,7 @@ __device__ void r_plan_body(const RArgs &a)
         c.tnext = tmin;
         if (c.nact > 0) w += (double)c.nact * c.n;
     }
-    // total work: per-wave xor tree, then the waves in order (fixed order)
+    // total work: per-wave DPP tree, then the waves in order (fixed order)
     w = wave_sum(w);
     if (lane == 0) s_wsum[wave] = w;
     __syncthreads();
@@ -1064,14 +1070,19 @@ __device__ __forceinline__ void r_force_body(const RArgs &a)
     }
 }
 
-// Two waves per SIMD: the 8th-order body alone would take 288 VGPRs (one
-// wave); held to 256 with a few spills it runs 4 % faster.  Three waves
-// (168 VGPRs) made the long steps of the 6th-order integration 7 % slower.
+// At least two waves per SIMD: the 8th-order body alone would take 288 VGPRs
+// (one wave); held to 256 with a few spills it runs 4 % faster.  The
+// 6th-order FP64 body is held to exactly two: left free, the compiler fits
+// it in 168 VGPRs for three waves, which made the long steps of the
+// 6th-order integration 7 % slower (and the whole C2 run 1-2 %).  The other
+// bodies keep the compiler's choice (4th order: 4 waves).
 //
 // STAMP: the timed instance writes the step's force stamp (hhr_set_timing
 // mode 2).
+__host__ __device__ constexpr int r_max_waves(int order, int mode) { return order == 6 && mode == FP64 ? 2 : 8; }
+
 template <int ORDER, int MODE, bool STAMP>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void r_force_kernel(RArgs a)
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, r_max_waves(ORDER, MODE)))) void r_force_kernel(RArgs a)
 {
     if constexpr (STAMP) r_stamp_glob(a, 2, 1);
     r_force_body<ORDER, MODE>(a);
@@ -1092,7 +1103,7 @@ __device__ int r_find_active(const RClu *clu, int nclu, int pos)
 }
 
 // Sum every active particle's slice partials (RLX lanes per particle, fixed
-// xor tree) and apply the corrector.  Grid-stride over the active list of
+// DPP tree) and apply the corrector.  Grid-stride over the active list of
 // all clusters, so the launch shape is fixed (graph-capturable).  Short
 // lists take a whole wave per particle (one round of loads per lane), long
 // ones 8 lanes per particle; the order of additions is fixed by the slice
@@ -1100,9 +1111,9 @@ __device__ int r_find_active(const RClu *clu, int nclu, int pos)
 template <int RLX>
 __device__ __forceinline__ double r_lane_tree(double x)
 {
-#pragma unroll
-    for (int off = RLX / 2; off >= 1; off >>= 1) x += __shfl_xor(x, off, 64);
-    return x;
+    static_assert(RLX == 8 || RLX == 64, "lane groups of 8 or a whole wave");
+    if constexpr (RLX == 8) return lane_tree8(x);
+    else return wave_sum(x);
 }
 
 template <int ORDER, int RLX>
@@ -1444,7 +1455,7 @@ __global__ __launch_bounds__(NT) void r_fused_force_kernel(RArgs a, int n, int o
                 }
                 if constexpr (MODE == SPLIT) split_fold<ORDER>(A[g], Af);
             }
-            // workgroup sums in a fixed order: shuffle tree per wave, waves
+            // workgroup sums in a fixed order: DPP tree per wave, waves
             // 0..3; the two red buffers alternate, one barrier per group
 #pragma unroll
             for (int g = 0; g < R_GI; g++)
@@ -1663,7 +1674,7 @@ __global__ __launch_bounds__(NT) void r_fused_islice_kernel(RArgs a, int n, int 
             }
             if constexpr (MODE == SPLIT) split_fold<ORDER>(A[g], Af);
         }
-        // workgroup sums in a fixed order: shuffle tree per wave, waves
+        // workgroup sums in a fixed order: DPP tree per wave, waves
         // 0..3; the two red buffers alternate, one barrier per group
 #pragma unroll
         for (int g = 0; g < R_GI; g++)
