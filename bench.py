@@ -605,8 +605,9 @@ def main():
         "micro": mic,
         "cpu_baseline": cpu,
     }
-    line["timing"] = ("one stamp-timed pass (device clock at every kernel start, graph replay)" if single else
-                      "untimed run, then a stamp-timed pass (device clock at every kernel start, graph replay) "
+    how = "graph replay" if graph else "direct launches"
+    line["timing"] = (f"one stamp-timed pass (device clock at every kernel start, {how})" if single else
+                      f"untimed run, then a stamp-timed pass (device clock at every kernel start, {how}) "
                       "for the roofline")
     if w.name == "c4":
         line["dE_E_median"] = f.get("dE_E_median")

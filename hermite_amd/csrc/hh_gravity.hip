@@ -827,16 +827,19 @@ static hipError_t r_launch_step_t(hhr_ctx *c, int s)
     const RArgs a = r_args(c);
     hipError_t e = r_mark(c, 4 * s);
     if (e != hipSuccess) return e;
+    // ensembles of up to R_PF_MAX clusters: planned inside the force launch
+    const bool pf = c->nclu > 1 && c->nclu <= R_PF_MAX;
     if (c->nclu == 1 && c->ntiles <= R_PACK_MAX) {  // one cluster: planned by the compaction's last workgroup
         hipLaunchKernelGGL((r_compact_kernel<ORDER, MODE == EXACT, 1, ST>), dim3(c->ntiles), dim3(NT), 0, c->stream, a);
     } else if (c->nclu == 1) {
         hipLaunchKernelGGL((r_compact_kernel<ORDER, MODE == EXACT, 2, ST>), dim3(c->ntiles), dim3(NT), 0, c->stream, a);
     } else {
         hipLaunchKernelGGL((r_compact_kernel<ORDER, MODE == EXACT, 0, ST>), dim3(c->ntiles), dim3(NT), 0, c->stream, a);
-        hipLaunchKernelGGL((r_plan_kernel<ORDER>), dim3(1), dim3(NT), 0, c->stream, a);
+        if (!pf) hipLaunchKernelGGL((r_plan_kernel<ORDER>), dim3(1), dim3(NT), 0, c->stream, a);
     }
     if ((e = r_mark(c, 4 * s + 1)) != hipSuccess) return e;
-    hipLaunchKernelGGL((r_force_kernel<ORDER, MODE, ST>), dim3(c->force_grid), dim3(NT), 0, c->stream, a);
+    if (pf) hipLaunchKernelGGL((r_force_kernel<ORDER, MODE, ST, true>), dim3(c->force_grid), dim3(NT), 0, c->stream, a);
+    else hipLaunchKernelGGL((r_force_kernel<ORDER, MODE, ST, false>), dim3(c->force_grid), dim3(NT), 0, c->stream, a);
     if ((e = r_mark(c, 4 * s + 2)) != hipSuccess) return e;
     hipLaunchKernelGGL((r_reduce_correct_kernel<ORDER, ST>), dim3(c->reduce_grid), dim3(NT), 0, c->stream, a);
     if ((e = r_mark(c, 4 * s + 3)) != hipSuccess) return e;

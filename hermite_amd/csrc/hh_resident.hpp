@@ -1,7 +1,8 @@
 // GPU-resident block-timestep integration of one or many independent
 // clusters (ensembles): the whole block step of SRC/nbodysystem.h:692-765
-// runs on the device, so a step costs four kernel launches (three for a
-// single cluster; capturable in a hipGraph) and no host round trip.
+// runs on the device, so a step costs three kernel launches (four for an
+// ensemble of more than R_PF_MAX clusters; capturable in a hipGraph) and no
+// host round trip.
 //
 // Per block step:
 //   r_compact_kernel  active list of every cluster (wave-aggregated append),
@@ -11,7 +12,8 @@
 //   r_plan_kernel     per-cluster dtlim, step counters, kernel choice and
 //                     geometry, task table of the persistent force kernel
 //                     (for a single cluster the compaction's last workgroup
-//                     does this instead)
+//                     does this instead, for an ensemble of at most
+//                     R_PF_MAX clusters the force kernel)
 //   r_force_kernel    persistent: a fixed grid walks the step's table of
 //                     (cluster, i-group, j-slice) tasks, writing partial forces
 //   r_reduce_correct_kernel
@@ -352,7 +354,7 @@ __device__ __forceinline__ double r_wave_min(double x)
 
 __device__ __forceinline__ unsigned long long r_bits(double t) { return (unsigned long long)__double_as_longlong(t); }
 
-template <int ORDER>
+template <int ORDER, bool PF = false>
 __device__ void r_plan_body(const RArgs &a);
 
 // Fields of a single cluster that its plan reads, loaded by every workgroup
@@ -599,18 +601,41 @@ __device__ __forceinline__ RGeom r_geometry(const RPlan &pl, bool exact, int nac
     return g;
 }
 
+// Total work of a step's clusters, w = nact * n of the calling thread's
+// clusters: a DPP tree per wave, then the waves in order (fixed order, so
+// every workgroup that computes it gets the same bits).
+__device__ __forceinline__ double r_total_work(double w, double *s_wsum)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    w = wave_sum(w);
+    if (lane == 0) s_wsum[wave] = w;
+    __syncthreads();
+    double total = 0.0;
+#pragma unroll
+    for (int q = 0; q < NT / 64; q++) total += s_wsum[q];
+    return total;
+}
+
 // One workgroup; thread t owns the clusters [t*cpt, (t+1)*cpt).  Geometry
 // per cluster from its active count; the task budget is shared in
 // proportion to each cluster's nact * n work; task, active-list and partial
 // offsets are block scans in cluster order.
-template <int ORDER>
+//
+// PF: the bookkeeping of a step whose force workgroups planned their tasks
+// themselves (r_pf_plan), run by one workgroup of that force launch.  The
+// active count is the sum of both parity counters (the one not in use is
+// zero: the reduce kernel clears a step's counter once every force
+// workgroup has read it), and a cluster whose step collapses keeps the
+// place of its tasks and partials, which the force workgroups planned
+// without that check.
+template <int ORDER, bool PF>
 __device__ void r_plan_body(const RArgs &a)
 {
     constexpr int NC = f_words(ORDER);
     constexpr int NW = NT / 64;
     __shared__ double s_wsum[NW];
     __shared__ long long s_scan[NW];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x;
     const int nclu = a.nclu;
     const int p = a.glob->step & 1;
     const bool exact = a.glob->exact != 0;
@@ -618,13 +643,18 @@ __device__ void r_plan_body(const RArgs &a)
     const int cpt = (nclu + NT - 1) / NT;
     const int k0 = min(tid * cpt, nclu), k1 = min(k0 + cpt, nclu);
 
-    // active counts and block-time minima of the owned clusters; both
-    // parity buffers of this step are cleared for their next use
+    // active counts and block-time minima of the owned clusters; this
+    // step's block-time slots (and, planned here, its counter) are cleared
+    // for their next use
     double w = 0.0;
     for (int k = k0; k < k1; k++) {
         RClu &c = a.clu[k];
-        c.nact = c.acnt[p];
-        c.acnt[p] = 0;
+        if constexpr (PF) {
+            c.nact = c.acnt[0] + c.acnt[1];
+        } else {
+            c.nact = c.acnt[p];
+            c.acnt[p] = 0;
+        }
         c.mode = 0;
         c.ntask = 0;
         double tmin = r_inf();
@@ -635,13 +665,7 @@ __device__ void r_plan_body(const RArgs &a)
         c.tnext = tmin;
         if (c.nact > 0) w += (double)c.nact * c.n;
     }
-    // total work: per-wave DPP tree, then the waves in order (fixed order)
-    w = wave_sum(w);
-    if (lane == 0) s_wsum[wave] = w;
-    __syncthreads();
-    double total = 0.0;
-#pragma unroll
-    for (int q = 0; q < NW; q++) total += s_wsum[q];
+    const double total = r_total_work(w, s_wsum);
 
     int my_tasks = 0, my_act = 0, my_live = 0;
     long long my_part = 0;
@@ -650,19 +674,23 @@ __device__ void r_plan_body(const RArgs &a)
         if (c.nact == 0) continue;
         const double tmin = c.tnext;
         const double dtlim = r_dtlim(tmin, c.dtmax);
-        if (!(tmin > c.tsys) || dtlim <= 0.0) {  // collapsed step: stop this cluster
+        const bool stop = !(tmin > c.tsys) || dtlim <= 0.0;  // collapsed step: stop this cluster
+        const double share = ((double)c.nact * c.n) / total;
+        const RGeom g = r_geometry(pl, exact, c.nact, c.ntiles, share);
+        if (stop) {
             c.nact = 0;
             c.tend = c.tsys;
             atomicExch(&a.glob->error, 1);
-            continue;
+            if constexpr (!PF) continue;
+        } else {
+            c.dtlim = dtlim;
+            c.num_step += c.nact;
+            c.num_bstep += 1;
+            c.tsys = tmin;
+            c.mode = g.mode;
+            my_act += c.nact;
+            my_live += 1;
         }
-        c.dtlim = dtlim;
-        c.num_step += c.nact;
-        c.num_bstep += 1;
-        c.tsys = tmin;
-        const double share = ((double)c.nact * c.n) / total;
-        const RGeom g = r_geometry(pl, exact, c.nact, c.ntiles, share);
-        c.mode = g.mode;
         c.wj = g.wj;
         c.ngrp = g.ngrp;
         c.tps = g.tps;
@@ -671,8 +699,6 @@ __device__ void r_plan_body(const RArgs &a)
         c.ipad = g.ipad;
         c.ntask = g.ngrp * g.nslice;
         my_tasks += c.ntask;
-        my_act += c.nact;
-        my_live += 1;
         my_part += (long long)c.nslice * NC * c.ipad;
     }
     // offsets in cluster order
@@ -687,10 +713,8 @@ __device__ void r_plan_body(const RArgs &a)
         c.act0 = (int)act;
         c.pofs = p0;
         t0 += c.ntask;
-        if (c.mode != 0) {
-            act += c.nact;
-            p0 += (long long)c.nslice * NC * c.ipad;
-        }
+        if (c.ntask > 0) p0 += (long long)c.nslice * NC * c.ipad;
+        if (c.mode != 0) act += c.nact;
     }
     if (tid == 0) {
         a.glob->total_tasks = (int)tot_tasks;
@@ -818,6 +842,81 @@ __device__ int r_find_cluster(const RClu *clu, int nclu, int task)
     return lo;
 }
 
+// Ensembles of at most R_PF_MAX clusters skip the plan launch: every
+// workgroup of the force kernel plans the step's tasks itself (the
+// geometry of r_plan_body, one thread per cluster, the same bits in every
+// workgroup) into LDS, and one workgroup then does the step's bookkeeping
+// (r_plan_body<ORDER, true>) for the reduce kernel.  LDS rows of the plan
+// (R_PF_MAX ints each, in the cluster-key array): task0, nact, mode | wj
+// << 4, nslice, tps or jchunk, ipad, pofs low and high words.
+constexpr int R_PF_MAX = NT;
+static_assert(8 * R_PF_MAX <= R_CLU_LDS, "plan rows in the cluster-key array");
+
+template <int ORDER>
+__device__ int r_pf_plan(const RArgs &a, int *s_key)
+{
+    constexpr int NC = f_words(ORDER);
+    __shared__ double s_wsum[NT / 64];
+    __shared__ long long s_scan[NT / 64];
+    const int tid = threadIdx.x, nclu = a.nclu;
+    const bool exact = a.glob->exact != 0;
+    const RPlan pl = a.glob->plan;
+    int nact = 0, n = 0, ntiles = 0;
+    if (tid < nclu) {
+        const RClu &c = a.clu[tid];
+        nact = c.acnt[0] + c.acnt[1];  // the counter not in use is zero
+        n = c.n;
+        ntiles = c.ntiles;
+    }
+    double w = 0.0;
+    if (nact > 0) w += (double)nact * n;
+    const double total = r_total_work(w, s_wsum);
+    RGeom g{};
+    long long mine = 0;  // partial words << 24 | tasks
+    if (nact > 0) {
+        const double share = ((double)nact * n) / total;
+        g = r_geometry(pl, exact, nact, ntiles, share);
+        mine = ((long long)g.nslice * NC * g.ipad << 24) | (long long)(g.ngrp * g.nslice);
+    }
+    long long all;
+    const long long before = r_block_exscan<long long>(mine, s_scan, &all);
+    if (tid < nclu) {
+        const long long pofs = before >> 24;
+        s_key[tid] = (int)(before & 0xffffff);
+        s_key[R_PF_MAX + tid] = nact;
+        s_key[2 * R_PF_MAX + tid] = g.mode | g.wj << 4;
+        s_key[3 * R_PF_MAX + tid] = g.nslice;
+        s_key[4 * R_PF_MAX + tid] = g.mode == 1 ? g.tps : g.jchunk;
+        s_key[5 * R_PF_MAX + tid] = g.ipad;
+        s_key[6 * R_PF_MAX + tid] = (int)(pofs & 0xffffffffLL);
+        s_key[7 * R_PF_MAX + tid] = (int)(pofs >> 32);
+    }
+    __syncthreads();
+    return (int)(all & 0xffffff);
+}
+
+// A task's cluster fields: the plan's from the LDS rows, the fixed ones
+// from the cluster record.
+__device__ __forceinline__ RCluF r_pf_fields(const RArgs &a, const int *s_key, int k)
+{
+    const RClu &c = a.clu[k];
+    RCluF f;
+    f.eps2 = c.eps2;
+    f.off = c.off;
+    f.n = c.n;
+    f.ntiles = c.ntiles;
+    f.task0 = s_key[k];
+    f.nact = s_key[R_PF_MAX + k];
+    const int mw = s_key[2 * R_PF_MAX + k];
+    f.mode = mw & 15;
+    f.wj = mw >> 4;
+    f.nslice = s_key[3 * R_PF_MAX + k];
+    f.tps = f.jchunk = s_key[4 * R_PF_MAX + k];
+    f.ipad = s_key[5 * R_PF_MAX + k];
+    f.pofs = (long long)(unsigned)s_key[6 * R_PF_MAX + k] | (long long)s_key[7 * R_PF_MAX + k] << 32;
+    return f;
+}
+
 // Persistent force kernel: a fixed grid of workgroups walks the step's task
 // table, which holds tasks of either kind.  Measured choices: one kernel
 // with both task kinds runs the tile tasks faster than a tile-only kernel
@@ -826,7 +925,7 @@ __device__ int r_find_cluster(const RClu *clu, int nclu, int task)
 // tile loop fits only by giving up scheduling freedom and the 8th-order one
 // only by spilling; the 8th-order kernel, left alone, would take 288
 // registers (one wave per SIMD).
-template <int ORDER, int MODE>
+template <int ORDER, int MODE, bool PF>
 __device__ __forceinline__ void r_force_body(const RArgs &a)
 {
     constexpr int Q = pr_words(ORDER);
@@ -850,12 +949,17 @@ __device__ __forceinline__ void r_force_body(const RArgs &a)
     // a single cluster's task fields are read together with the task count
     // (one round trip to memory instead of two)
     RCluF c0{};
-    if (nclu == 1) c0 = r_clu_fields(a.clu[0]);
-    const int total = a.glob->total_tasks;
-    const bool keys_lds = nclu > 1 && nclu <= R_CLU_LDS;
-    if (keys_lds) {
-        for (int k = tid; k < nclu; k += NT) s_key[k] = a.clu[k].task0;
-        __syncthreads();
+    if (!PF && nclu == 1) c0 = r_clu_fields(a.clu[0]);
+    const bool keys_lds = !PF && nclu > 1 && nclu <= R_CLU_LDS;
+    int total;
+    if constexpr (PF) {
+        total = r_pf_plan<ORDER>(a, s_key);
+    } else {
+        total = a.glob->total_tasks;
+        if (keys_lds) {
+            for (int k = tid; k < nclu; k += NT) s_key[k] = a.clu[k].task0;
+            __syncthreads();
+        }
     }
 
     // Static round-robin over the task table: no queue counter (a shared
@@ -864,9 +968,10 @@ __device__ __forceinline__ void r_force_body(const RArgs &a)
     // grid that is a multiple of 8 task t runs on XCD t % 8, so slices that
     // are a multiple of 8 apart share an L2.
     for (int task = blockIdx.x; task < total; task += gridDim.x) {
-        const RCluF c = nclu == 1 ? c0
-                                  : r_clu_fields(a.clu[keys_lds ? r_find_lds(s_key, nclu, task)
-                                                                : r_find_cluster(a.clu, nclu, task)]);
+        const RCluF c = PF          ? r_pf_fields(a, s_key, r_find_lds(s_key, nclu, task))
+                        : nclu == 1 ? c0
+                                    : r_clu_fields(a.clu[keys_lds ? r_find_lds(s_key, nclu, task)
+                                                                  : r_find_cluster(a.clu, nclu, task)]);
         const int local = task - c.task0;
         const int grp = local / c.nslice, slice = local - grp * c.nslice;
         const double eps2 = c.eps2;
@@ -1078,14 +1183,23 @@ __device__ __forceinline__ void r_force_body(const RArgs &a)
 // bodies keep the compiler's choice (4th order: 4 waves).
 //
 // STAMP: the timed instance writes the step's force stamp (hhr_set_timing
-// mode 2).
+// mode 2).  PF: an ensemble of at most R_PF_MAX clusters, planned by the
+// force workgroups themselves (r_pf_plan); workgroup 0 first does the
+// step's bookkeeping and advances the step counter, which no other
+// workgroup of this launch reads (done after its tasks, the bookkeeping
+// made the 4th-order kernel spill).  At 4th order the kernel is held to the
+// four waves per SIMD that the compiler gives it without the plan.
 __host__ __device__ constexpr int r_max_waves(int order, int mode) { return order == 6 && mode == FP64 ? 2 : 8; }
+__host__ __device__ constexpr int r_min_waves(int order, bool pf) { return pf && order == 4 ? 4 : 2; }
 
-template <int ORDER, int MODE, bool STAMP>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, r_max_waves(ORDER, MODE)))) void r_force_kernel(RArgs a)
+template <int ORDER, int MODE, bool STAMP, bool PF>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(r_min_waves(ORDER, PF), r_max_waves(ORDER, MODE)))) void r_force_kernel(RArgs a)
 {
-    if constexpr (STAMP) r_stamp_glob(a, 2, 1);
-    r_force_body<ORDER, MODE>(a);
+    if constexpr (STAMP) r_stamp_glob(a, 2, PF ? 0 : 1);
+    if constexpr (PF) {
+        if (blockIdx.x == 0) r_plan_body<ORDER, true>(a);
+    }
+    r_force_body<ORDER, MODE, PF>(a);
 }
 
 // Cluster of an active-list position: the last cluster whose act0 <= pos
@@ -1178,6 +1292,13 @@ __global__ __launch_bounds__(NT) void r_reduce_correct_kernel(RArgs a)
         ipRL = a.ilist[min((int)(blockIdx.x * (NT / RL) + threadIdx.x / RL), a.npad - 1)];
     }
     const int total = a.glob->total_active;
+    // an ensemble planned in its force launch: clear the step's append
+    // counters now that every force workgroup has read them (the step
+    // counter has advanced: they are those of the other parity)
+    if (nclu > 1 && nclu <= R_PF_MAX && blockIdx.x == 0) {
+        const int pc = (a.glob->step & 1) ^ 1;
+        for (int k = threadIdx.x; k < nclu; k += NT) a.clu[k].acnt[pc] = 0;
+    }
     const bool wide = total <= R_WIDE_MAX;
     if (blockIdx.x * (wide ? NT / 64 : NT / RL) >= total) return;  // whole workgroup idle this step
     const bool keys_lds = nclu > 1 && nclu <= R_CLU_LDS;

@@ -382,6 +382,66 @@ def test_large_ensemble_bitwise_vs_oracle_inplace():
     r.close()
 
 
+def test_ensemble_planned_in_force_launch_bitwise():
+    """Ensembles of up to 256 clusters are planned inside the force launch
+    (every workgroup plans the step's tasks itself, one of them does the
+    bookkeeping); larger ones by the plan kernel.  256 and 257 clusters of 3
+    to 12 particles, EXACT: each of the first 256 clusters ends with the
+    same records and counts in both ensembles, and a sample equals the
+    oracle's in-place schedule of that cluster alone."""
+    order, eps2, eta, eta_s = 6, (1 / 32) ** 2, 0.4, 0.1
+    sizes = [3 + k % 10 for k in range(257)]
+    systems = [initialised(order, n, 3000 + k, eps2, eta, eta_s, 1 / 16) for k, n in enumerate(sizes)]
+    clus = [(o.ptcl(), eps2, eta, 1 / 16, 0.0, 0.0) for o in systems]
+    print("resident ensembles of 256 and 257 clusters", flush=True)
+    a = H.Resident(order, "exact", clus[:256])
+    b = H.Resident(order, "exact", clus)
+    a.run(1 / 16, use_graph=True, batch=16)
+    b.run(1 / 16, use_graph=True, batch=16)
+    for k in range(256):
+        assert a.stats(k) == b.stats(k), k
+        assert np.array_equal(a.particles(k), b.particles(k)), k
+    for k in range(0, 257, 32):
+        o = systems[k]
+        o.integrate_until_inplace(1 / 16)
+        st = b.stats(k)
+        assert (st["num_step"], st["num_bstep"], st["tsys"]) == (o.num_step, o.num_bstep, o.tsys), k
+        assert np.array_equal(b.particles(k), o.ptcl()), k
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_ensemble_planned_in_force_launch_timing(use_graph):
+    """The stamp-timed instances of an ensemble planned inside its force
+    launch (the step counter advances during that launch) leave the
+    integration bit-identical and log every block step."""
+    order, eta, eta_s, dtmax = 4, 0.1, 0.01, 1 / 16
+    clusters = []
+    for k, n in enumerate([300, 700, 1024]):
+        eps2 = (4.0 / n) ** 2
+        o = initialised(order, n, 400 + k, eps2, eta, eta_s, dtmax)
+        clusters.append((o.ptcl(), eps2, eta, dtmax, 0.0, 0.0))
+    print(f"ensemble timing graph={use_graph}", flush=True)
+    a = H.Resident(order, "fp64", clusters)
+    b = H.Resident(order, "fp64", clusters)
+    a.run(1 / 8, use_graph=use_graph, batch=8)
+    b.set_timing(True, stamps=True)
+    b.run(1 / 8, use_graph=use_graph, batch=8)
+    ms, launches = b.force_timing()
+    na, fm, om = b.step_log()
+    for k in range(3):
+        assert a.stats(k) == b.stats(k), k
+        assert np.array_equal(a.particles(k), b.particles(k)), k
+    assert launches == b.steps_launched
+    nb = [b.stats(k)["num_bstep"] for k in range(3)]
+    assert max(nb) <= len(na) <= sum(nb)
+    assert (na > 0).all() and (fm > 0).all() and (om > 0).all()
+    assert 0 < ms < 1e4
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize("order", ORDERS)
 @pytest.mark.parametrize("variant", ["fp64", "split"])
 def test_fused_step_equals_planned_step(order, variant):
