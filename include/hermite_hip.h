@@ -276,7 +276,7 @@ int hhr_force_timing(const hhr_ctx *ctx, double *ms_total, long long *launches);
 /* Per-step log of the timed steps since timing was switched on (no reference
  * counterpart; the reference's Profile, SRC/nbodysystem.h:57-194, keeps
  * phase totals only): active particles over all clusters, force-kernel ms
- * and ms of the step's other three launches, for every step that had
+ * and ms of the step's other launches, for every step that had
  * active particles.  Copies min(cap, count) entries and returns count. */
 long long hhr_step_log(const hhr_ctx *ctx, long long cap, int *nact, float *force_ms, float *other_ms);
 /* Reference Particle records of cluster k (ns[k] * hh_particle_words). */
