@@ -6,7 +6,8 @@
 // the built-in one by 1.02x or more are rows; for these the built-in plan
 // was kept (within run-to-run noise of the best found):
 //   order 4, N 1024, 1 cluster(s) (c1, fp64): best {128, 512, 8192, 256, 8192, 16384} measured 1.003x the built-in plan
-//   order 4, N 549993, 256 cluster(s) (c4, fp64): best {64, 128, 8192, 512, 16384, 16384} measured 1.002x the built-in plan
+//   order 4, N 549993, 256 cluster(s) (c4, fp64): best {64, 128, 8192, 512, 8192, 8192} measured 1.003x the built-in plan
+//   order 4, N 549993, 256 cluster(s) (c4, split): best {64, 128, 8192, 512, 16384, 32768} measured 1.003x the built-in plan
 //   order 6, N 65536, 1 cluster(s) (c2, fp64): best {128, 512, 8192, 256, 8192, 16384} measured 1.008x the built-in plan
 //   order 6, N 1048576, 1 cluster(s) (c5, fp64): best {64, 128, 8192, 256, 8192, 16384} measured 1.002x the built-in plan
 //   order 8, N 262144, 1 cluster(s) (c3, fp64): best {128, 512, 8192, 256, 8192, 16384} measured 1.002x the built-in plan
