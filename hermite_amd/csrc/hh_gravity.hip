@@ -1316,8 +1316,10 @@ double hhr_last_run_ms(const hhr_ctx *c) { return c ? c->last_run_ms : -1.0; }
 int hhr_set_plan(hhr_ctx *c, int jpar_max, int smax, int task_target, int jpar_target)
 {
     if (!c) return HH_E_ARG;
-    if (jpar_max < 0 || smax < 1 || task_target < 1 || task_target > R_TASK_MAX || jpar_target < 1 ||
-        jpar_target > R_TASK_MAX)
+    // j-parallel groups are for short lists: at most R_JPAR_MAX / 2 groups
+    // per cluster keeps every step's task count far inside the task fields
+    if (jpar_max < 0 || jpar_max > R_JPAR_MAX || smax < 1 || task_target < 1 || task_target > R_TASK_MAX ||
+        jpar_target < 1 || jpar_target > R_TASK_MAX)
         return rfail(c, HH_E_ARG, "plan parameters out of range");
     RCHK(c, hipSetDevice(c->device));
     RCHK(c, hipStreamSynchronize(c->stream));

@@ -45,6 +45,7 @@ constexpr int R_GI = 2;          // i per group of the j-parallel task (half the
 // (halving LDS traffic per interaction); the EXACT variant keeps one
 __host__ __device__ constexpr int r_ni(int mode) { return mode == EXACT ? 1 : 2; }
 constexpr int R_TASK_MAX = 8192; // largest task target a plan may ask for
+constexpr int R_JPAR_MAX = 4096; // largest active count a plan may give j-parallel groups
 constexpr int R_SLOTS = 32;      // words a block-time minimum is spread over
 // Steps with at most this many active particles sum each particle's slices
 // on a whole wave (one round of loads); larger ones on 8 lanes
@@ -872,7 +873,10 @@ __device__ int r_pf_plan(const RArgs &a, int *s_key)
     if (nact > 0) w += (double)nact * n;
     const double total = r_total_work(w, s_wsum);
     RGeom g{};
-    long long mine = 0;  // partial words << 24 | tasks
+    // partial words << 24 | tasks: a step has fewer than 2^24 tasks (the
+    // budgets plus one per group, with at most R_JPAR_MAX / 2 j-parallel
+    // groups per cluster and 128 i or more per i-parallel group)
+    long long mine = 0;
     if (nact > 0) {
         const double share = ((double)nact * n) / total;
         g = r_geometry(pl, exact, nact, ntiles, share);

@@ -232,7 +232,7 @@ int hhr_set_step_limit(hhr_ctx *ctx, long long max_block_steps);
  * i-parallel groups of 512 i (128 or 256 when tiles are split over waves,
  * hhr_set_split_waves) with at most smax j-slices; a block step aims at
  * task_target i-parallel (jpar_target j-parallel) tasks over all clusters
- * (both at most 8192).  hhr_create starts from the measured plan of the
+ * (both at most 8192; jpar_max at most 4096).  hhr_create starts from the measured plan of the
  * same order and kind (one cluster or an ensemble) nearest in total
  * particles, within a factor of 4, where the autotuner found one at least
  * 2 % faster than the built-in plan (hermite_amd/csrc/hh_plan_table.hpp),

@@ -362,6 +362,21 @@ def test_plan_and_split_wave_settings_are_independent():
     assert not np.array_equal(out[0][1], out[2][1])
 
 
+def test_plan_parameters_out_of_range_are_rejected():
+    """hhr_set_plan refuses task targets above 8192 and j-parallel groups for
+    more than 4096 active particles, and the context still runs."""
+    order, n, eps2 = 4, 300, (1 / 64) ** 2
+    o = initialised(order, n, 68, eps2, 0.1, 0.01, 1 / 16)
+    r = H.Resident(order, "fp64", [(o.ptcl(), eps2, 0.1, 1 / 16, 0.0, 0.0)])
+    for bad in ((4097, 256, 8192, 256), (64, 256, 8193, 256), (64, 256, 8192, 8193), (64, 0, 8192, 256)):
+        with pytest.raises(H.HHError):
+            r.set_plan(*bad)
+    r.set_plan(4096, 256, 8192, 256)
+    r.run(1 / 16)
+    assert r.stats()["num_bstep"] > 0
+    r.close()
+
+
 def test_large_ensemble_bitwise_vs_oracle_inplace():
     """2100 clusters of 3 to 12 particles: more clusters than the plan
     kernel has threads (several per thread) and than the force and reduce
