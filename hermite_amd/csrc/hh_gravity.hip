@@ -838,7 +838,8 @@ static hipError_t r_launch_step_t(hhr_ctx *c, int s)
         if (!pf) hipLaunchKernelGGL((r_plan_kernel<ORDER>), dim3(1), dim3(NT), 0, c->stream, a);
     }
     if ((e = r_mark(c, 4 * s + 1)) != hipSuccess) return e;
-    if (pf) hipLaunchKernelGGL((r_force_kernel<ORDER, MODE, ST, true>), dim3(c->force_grid), dim3(NT), 0, c->stream, a);
+    if (pf)  // one more workgroup, for the step's bookkeeping
+        hipLaunchKernelGGL((r_force_kernel<ORDER, MODE, ST, true>), dim3(c->force_grid + 1), dim3(NT), 0, c->stream, a);
     else hipLaunchKernelGGL((r_force_kernel<ORDER, MODE, ST, false>), dim3(c->force_grid), dim3(NT), 0, c->stream, a);
     if ((e = r_mark(c, 4 * s + 2)) != hipSuccess) return e;
     hipLaunchKernelGGL((r_reduce_correct_kernel<ORDER, ST>), dim3(c->reduce_grid), dim3(NT), 0, c->stream, a);

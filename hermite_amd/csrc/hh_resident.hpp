@@ -971,7 +971,9 @@ __device__ __forceinline__ void r_force_body(const RArgs &a)
     // near-equal work (the plan shares the budget by nact * n), and with a
     // grid that is a multiple of 8 task t runs on XCD t % 8, so slices that
     // are a multiple of 8 apart share an L2.
-    for (int task = blockIdx.x; task < total; task += gridDim.x) {
+    // PF: workgroup 0 does the step's bookkeeping only; the tasks go to the others
+    const int tfirst = PF ? (int)blockIdx.x - 1 : (int)blockIdx.x, tstride = PF ? (int)gridDim.x - 1 : (int)gridDim.x;
+    for (int task = tfirst; task < total; task += tstride) {
         const RCluF c = PF          ? r_pf_fields(a, s_key, r_find_lds(s_key, nclu, task))
                         : nclu == 1 ? c0
                                     : r_clu_fields(a.clu[keys_lds ? r_find_lds(s_key, nclu, task)
@@ -1188,11 +1190,11 @@ __device__ __forceinline__ void r_force_body(const RArgs &a)
 //
 // STAMP: the timed instance writes the step's force stamp (hhr_set_timing
 // mode 2).  PF: an ensemble of at most R_PF_MAX clusters, planned by the
-// force workgroups themselves (r_pf_plan); workgroup 0 first does the
-// step's bookkeeping and advances the step counter, which no other
-// workgroup of this launch reads (done after its tasks, the bookkeeping
-// made the 4th-order kernel spill).  At 4th order the kernel is held to the
-// four waves per SIMD that the compiler gives it without the plan.
+// force workgroups themselves (r_pf_plan); workgroup 0, launched beside
+// the grid of task workgroups, does the step's bookkeeping instead of
+// tasks and advances the step counter, which no other workgroup of this
+// launch reads.  At 4th order the kernel is held to the four waves per SIMD
+// that the compiler gives it without the plan.
 __host__ __device__ constexpr int r_max_waves(int order, int mode) { return order == 6 && mode == FP64 ? 2 : 8; }
 __host__ __device__ constexpr int r_min_waves(int order, bool pf) { return pf && order == 4 ? 4 : 2; }
 
@@ -1201,7 +1203,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(r_min_waves(
 {
     if constexpr (STAMP) r_stamp_glob(a, 2, PF ? 0 : 1);
     if constexpr (PF) {
-        if (blockIdx.x == 0) r_plan_body<ORDER, true>(a);
+        if (blockIdx.x == 0) {
+            r_plan_body<ORDER, true>(a);
+            return;
+        }
     }
     r_force_body<ORDER, MODE, PF>(a);
 }
