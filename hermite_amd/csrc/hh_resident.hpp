@@ -126,15 +126,18 @@ struct RArgs {
 
 // Step timing inside replayed graphs (hhr_set_timing mode 2): the first
 // thread of the first workgroup of each kernel of block step s writes the
-// device's constant-rate clock, read at kernel start, to
-// stamp[(s % cap) * 4 + k]: k = 0 compaction (the fused force kernel: the
-// step's first launch), 2 force, 3 corrector.  The launches of a graph
-// follow each other with no host in between, so the next step's first
-// stamp ends a step.  Kernels that may advance the step counter themselves
-// pass the value they read before their own atomics.  Each kernel has a
-// timed instance (template flag STAMP) and an untimed one without this
-// code, which would change the untimed kernels' register allocation and
-// cost them 1-3 % (measured).
+// device's constant-rate clock to stamp[(s % cap) * 4 + k]: k = 0
+// compaction (the fused force kernel: the step's first launch), 2 force,
+// 3 corrector.  The clock is read as soon as the kernel's first loads are
+// issued, not before them: the clock read and the stamp's store count as
+// memory side effects, and ahead of the loads they make the compiler turn
+// uniform scalar loads into vector loads that wait one after the other.
+// The launches of a graph follow each other with no host in between, so
+// the next step's first stamp ends a step.  Kernels that may advance the
+// step counter themselves pass the value they read before their own
+// atomics.  Each kernel has a timed instance (template flag STAMP) and an
+// untimed one without this code, which would change the untimed kernels'
+// register allocation and cost them 1-3 % (measured).
 
 __device__ __forceinline__ void r_stamp(const RArgs &a, int k, int step, long long t)
 {
@@ -404,7 +407,6 @@ template <int ORDER, bool EX, int CM, bool STAMP>
 __global__ __launch_bounds__(NT) void r_compact_kernel(RArgs a)
 {
     constexpr bool ONE = CM != 0, packed = CM == 1;
-    const long long t_start = STAMP ? wall_clock64() : 0;
     constexpr int W = jp_words(ORDER), Q = pr_words(ORDER);
     constexpr int NW = NT / 64;
     __shared__ double wmin[NW];
@@ -416,7 +418,6 @@ __global__ __launch_bounds__(NT) void r_compact_kernel(RArgs a)
     RClu &c = a.clu[k];
     const int i = tile * TJ + tid;
     const int step = a.glob->step;
-    if constexpr (STAMP) r_stamp(a, 0, step, t_start);
     const unsigned long long none = 0x7ff0000000000000ULL;
     const unsigned long long b0 = lane < R_SLOTS ? c.tmin_slot[0][lane] : none;
     const unsigned long long b1 = lane < R_SLOTS ? c.tmin_slot[1][lane] : none;
@@ -439,6 +440,14 @@ __global__ __launch_bounds__(NT) void r_compact_kernel(RArgs a)
     if constexpr (packed) {
 #pragma unroll
         for (int w = 0; w < W; w++) r[w] = a.rp[(size_t)w * a.npad + i];
+    }
+    // The timed instance reads the clock (workgroup 0) and stores the stamp
+    // only once the loads above are issued: the clock read and the store
+    // count as memory side effects, and placed before those loads they
+    // turned the uniform ones into vector loads waited for one after the
+    // other (C3 compaction 48-51 us instead of 21).
+    if constexpr (STAMP) {
+        if (blockIdx.x == 0) r_stamp(a, 0, step, wall_clock64());
     }
     const int p = step & 1;
     const double tmin = r_wave_min(__longlong_as_double((long long)(p ? b1 : b0)));
@@ -929,7 +938,7 @@ __device__ __forceinline__ RCluF r_pf_fields(const RArgs &a, const int *s_key, i
 // tile loop fits only by giving up scheduling freedom and the 8th-order one
 // only by spilling; the 8th-order kernel, left alone, would take 288
 // registers (one wave per SIMD).
-template <int ORDER, int MODE, bool PF>
+template <int ORDER, int MODE, bool PF, bool STAMP>
 __device__ __forceinline__ void r_force_body(const RArgs &a)
 {
     constexpr int Q = pr_words(ORDER);
@@ -964,6 +973,9 @@ __device__ __forceinline__ void r_force_body(const RArgs &a)
             for (int k = tid; k < nclu; k += NT) s_key[k] = a.clu[k].task0;
             __syncthreads();
         }
+        // the step's force stamp, once the loads above are issued (see
+        // r_compact_kernel)
+        if constexpr (STAMP) r_stamp_glob(a, 2, 1);
     }
 
     // Static round-robin over the task table: no queue counter (a shared
@@ -1201,14 +1213,14 @@ __host__ __device__ constexpr int r_min_waves(int order, bool pf) { return pf &&
 template <int ORDER, int MODE, bool STAMP, bool PF>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(r_min_waves(ORDER, PF), r_max_waves(ORDER, MODE)))) void r_force_kernel(RArgs a)
 {
-    if constexpr (STAMP) r_stamp_glob(a, 2, PF ? 0 : 1);
     if constexpr (PF) {
         if (blockIdx.x == 0) {
+            if constexpr (STAMP) r_stamp_glob(a, 2, 0);
             r_plan_body<ORDER, true>(a);
             return;
         }
     }
-    r_force_body<ORDER, MODE, PF>(a);
+    r_force_body<ORDER, MODE, PF, STAMP>(a);
 }
 
 // Cluster of an active-list position: the last cluster whose act0 <= pos
@@ -1287,7 +1299,6 @@ __device__ __forceinline__ void r_reduce_body(const RArgs &a, const int *s_key, 
 template <int ORDER, bool STAMP>
 __global__ __launch_bounds__(NT) void r_reduce_correct_kernel(RArgs a)
 {
-    if constexpr (STAMP) r_stamp_glob(a, 3, 1);
     __shared__ int s_key[R_CLU_LDS];
     const int nclu = a.nclu;
     // a single cluster's fields, and this thread's first active-list entry
@@ -1301,12 +1312,15 @@ __global__ __launch_bounds__(NT) void r_reduce_correct_kernel(RArgs a)
         ipRL = a.ilist[min((int)(blockIdx.x * (NT / RL) + threadIdx.x / RL), a.npad - 1)];
     }
     const int total = a.glob->total_active;
+    const int nb = a.glob->step & 1;  // the plan has advanced the step: next buffers
+    // the step's stamp, and the stores below, come after the loads above
+    // are issued (see r_compact_kernel)
+    if constexpr (STAMP) r_stamp_glob(a, 3, 1);
     // an ensemble planned in its force launch: clear the step's append
-    // counters now that every force workgroup has read them (the step
-    // counter has advanced: they are those of the other parity)
+    // counters now that every force workgroup has read them (those of the
+    // other parity than the advanced step counter's)
     if (nclu > 1 && nclu <= R_PF_MAX && blockIdx.x == 0) {
-        const int pc = (a.glob->step & 1) ^ 1;
-        for (int k = threadIdx.x; k < nclu; k += NT) a.clu[k].acnt[pc] = 0;
+        for (int k = threadIdx.x; k < nclu; k += NT) a.clu[k].acnt[nb ^ 1] = 0;
     }
     const bool wide = total <= R_WIDE_MAX;
     if (blockIdx.x * (wide ? NT / 64 : NT / RL) >= total) return;  // whole workgroup idle this step
@@ -1315,7 +1329,6 @@ __global__ __launch_bounds__(NT) void r_reduce_correct_kernel(RArgs a)
         for (int k = threadIdx.x; k < nclu; k += NT) s_key[k] = a.clu[k].act0;
         __syncthreads();
     }
-    const int nb = a.glob->step & 1;  // the plan has advanced the step: next buffers
     // each lane keeps the minimum of the new block times it produced for
     // one cluster and flushes it when the cluster changes
     double tacc = r_inf();
@@ -1469,7 +1482,6 @@ __device__ void r_fused_bookkeep(const RArgs &a, RClu &c, const RFusedStep &st, 
 template <int ORDER, int MODE, int JPL, bool STAMP>
 __global__ __launch_bounds__(NT) void r_fused_force_kernel(RArgs a, int n, int off, int S, int wps)
 {
-    if constexpr (STAMP) r_stamp_glob(a, 0, 0);  // also the force stamp: the step's first launch
     static_assert(MODE != EXACT, "EXACT sums j in index order: three-launch step");
     constexpr int W = jp_words(ORDER), Q = pr_words(ORDER);
     constexpr int NC = f_words(ORDER);
@@ -1501,6 +1513,9 @@ __global__ __launch_bounds__(NT) void r_fused_force_kernel(RArgs a, int n, int o
         for (int w = 0; w < W; w++) rj[u][w] = j < n ? a.rp[(size_t)w * a.npad + off + j] : 0.0;
     }
     const RFusedStep st = r_fused_step(a, c);
+    // the step's first stamp, also the force stamp, once the loads above
+    // are issued (see r_compact_kernel)
+    if constexpr (STAMP) r_stamp_glob(a, 0, 0);
 
     // active list in slot order
     unsigned long long amask[R_FUSED_TV];
@@ -1611,7 +1626,6 @@ __global__ __launch_bounds__(NT) void r_fused_force_kernel(RArgs a, int n, int o
 template <int ORDER, bool STAMP>
 __global__ __launch_bounds__(NT) void r_fused_correct_kernel(RArgs a, int n, int off, int S)
 {
-    const long long t_start = STAMP ? wall_clock64() : 0;
     constexpr int NC = f_words(ORDER), RW = r_words(ORDER);
     constexpr int NW = NT / 64;
     __shared__ double s_wmin[NW];
@@ -1636,7 +1650,10 @@ __global__ __launch_bounds__(NT) void r_fused_correct_kernel(RArgs a, int n, int
             for (int q = 0; q < NC; q++) f[q] += a.part[((size_t)sl * R_FUSED_MAX + i) * NC + q];
     const double etapow = c.etapow, dtmin = c.dtmin;
     const RFusedStep st = r_fused_step(a, c);
-    if constexpr (STAMP) r_stamp(a, 3, st.step, t_start);
+    // the stamp once the loads above are issued (see r_compact_kernel)
+    if constexpr (STAMP) {
+        if (blockIdx.x == 0) r_stamp(a, 3, st.step, wall_clock64());
+    }
     const bool act = st.stepping && told == st.tmin;
     double tnew = told;
     if (act) {  // r_finish on the words in registers
@@ -1701,7 +1718,6 @@ __global__ __launch_bounds__(NT) void r_fused_correct_kernel(RArgs a, int n, int
 template <int ORDER, int MODE, int JPL, bool STAMP>
 __global__ __launch_bounds__(NT) void r_fused_islice_kernel(RArgs a, int n, int off, int Si, int Sj, int wpc)
 {
-    if constexpr (STAMP) r_stamp_glob(a, 0, 0);  // also the force stamp: the step's first launch
     static_assert(MODE != EXACT, "EXACT sums j in index order: three-launch step");
     constexpr int W = jp_words(ORDER), Q = pr_words(ORDER);
     constexpr int NC = f_words(ORDER);
@@ -1733,6 +1749,9 @@ __global__ __launch_bounds__(NT) void r_fused_islice_kernel(RArgs a, int n, int 
         for (int w = 0; w < W; w++) rj[u][w] = j < n ? a.rp[(size_t)w * a.npad + off + j] : 0.0;
     }
     const RFusedStep st = r_fused_step(a, c);
+    // the step's first stamp, also the force stamp, once the loads above
+    // are issued (see r_compact_kernel)
+    if constexpr (STAMP) r_stamp_glob(a, 0, 0);
 
     // the i-slice's active particles, predicted, in slot order
     const bool act = st.stepping && ti == st.tmin;
