@@ -48,3 +48,16 @@ def test_plummer_state_matches_generator():
     P = W.cluster_state(6, small)
     m, x, v = H.plummer(512, c.seed)
     assert np.array_equal(P[:, 4:7], x) and np.array_equal(P[:, 7:10], v) and np.array_equal(P[:, 1], m)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus above 1 exits with a message (and
+    touches no device)."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode != 0
+    assert "one GPU" in (r.stdout + r.stderr)
