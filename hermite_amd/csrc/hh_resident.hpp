@@ -398,9 +398,10 @@ __device__ void r_plan_one(const RArgs &a, RClu &c, const RPlanIn &in, int step,
 // last workgroup to finish plans the step from registers (r_plan_one).
 // That saves the plan kernel's launch and every dependent read but the
 // atomic.  Ensembles append with one atomic per tile to their cluster's
-// counter and plan in r_plan_kernel (planning in the last workgroup, with
-// the workgroups counting themselves done in two levels of 64 counters,
-// made the 256-cluster ensemble 2.7 % slower on MI355X).
+// counter and are planned in the force launch (up to R_PF_MAX clusters) or
+// in r_plan_kernel (planning in the compaction's last workgroup, with the
+// workgroups counting themselves done in two levels of 64 counters, made
+// the 256-cluster ensemble 2.7 % slower on MI355X).
 // CM: 0 ensemble, 1 one cluster of at most R_PACK_MAX tiles, 2 one larger
 // cluster.
 template <int ORDER, bool EX, int CM, bool STAMP>
@@ -1312,7 +1313,7 @@ __global__ __launch_bounds__(NT) void r_reduce_correct_kernel(RArgs a)
         ipRL = a.ilist[min((int)(blockIdx.x * (NT / RL) + threadIdx.x / RL), a.npad - 1)];
     }
     const int total = a.glob->total_active;
-    const int nb = a.glob->step & 1;  // the plan has advanced the step: next buffers
+    const int nb = a.glob->step & 1;  // the step's plan has advanced the counter: next buffers
     // the step's stamp, and the stores below, come after the loads above
     // are issued (see r_compact_kernel)
     if constexpr (STAMP) r_stamp_glob(a, 3, 1);
